@@ -1,0 +1,170 @@
+"""Benchmark: exact-solved positions/sec at 16 empties on one MI355X.
+
+One step = one exact solve of the whole batch of random-playout positions
+(default 100,000 at 16 empties), boards already resident in device memory
+(a go2 task uploaded once), timed from launch to completion.  The headline
+solver (learned move ordering when a trained table is present, else
+fastest-first mobility ordering) is compared with the plain batched kernel
+(static corner / X-square ordering) on the same GPU, and every score is
+checked: headline vs plain on the plain sample, headline vs the CPU oracle
+on the CPU sample.
+
+Prints ONE JSON line (see the keys below).  `--gpus N` with N > 1 exits:
+this solver runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import gpuothello2_amd as go2  # noqa: E402
+
+# Integer work per search node of the reference formulation (one mobility
+# ~131 u64 ops + one flip ~82 + ~10 bookkeeping = 223 u64 ops = 446 int32
+# lane-ops): the algorithmic unit of the VALU roofline.
+ALGO_INT_OPS_PER_NODE = 446
+# MI355X: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz peak engine clock.
+PEAK_INT32_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
+
+
+def log(msg: str) -> None:
+    sys.stderr.write(msg + "\n")
+    sys.stderr.flush()
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=100000, help="positions per step")
+    ap.add_argument("--empties", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--ordering", choices=["best", "mobility", "learned", "static"], default="best")
+    ap.add_argument("--plain-n", type=int, default=2048, help="positions for the plain-kernel leg")
+    ap.add_argument("--cpu-sample", type=int, default=192, help="positions for the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
+    ap.add_argument("--skip-cpu", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this solver runs on one GPU; multi-GPU runs are not part of this build "
+              "(positions shard with no exchange, see the design notes).", file=sys.stderr)
+        return 2
+
+    E, n = args.empties, args.n
+    order_name = args.ordering
+    if order_name == "best":
+        order_name = "mobility"
+    ordering = {"static": go2.ORDER_STATIC, "mobility": go2.ORDER_MOBILITY, "learned": go2.ORDER_LEARNED}[order_name]
+    cfg = go2.default_config(ordering=ordering)
+
+    log(f"[bench] generating {n} positions at {E} empties on the GPU")
+    boards = go2.generate(args.seed, 4, n, E)
+    task = go2.Task(n, E, cfg)
+    task.upload(boards)
+
+    for i in range(args.warmup):
+        task.launch()
+        task.wait()
+        log(f"[bench] warmup {i + 1}/{args.warmup} done")
+    kernel_ms, nodes = [], []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        task.launch()
+        task.wait()
+        scores, st = task.download()
+        kernel_ms.append(st["kernel_ms"])
+        nodes.append(st["nodes"])
+        log(f"[bench] step {i + 1}/{args.steps}: kernel {st['kernel_ms']:.1f} ms, nodes {st['nodes']}")
+    t1 = time.perf_counter()
+    # download() is outside the solve; subtract nothing: it copies n bytes.
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = n / (ms_per_step / 1e3)
+    k_ms = float(np.mean(kernel_ms))
+    nodes_per_launch = float(np.mean(nodes))
+    achieved = nodes_per_launch * ALGO_INT_OPS_PER_NODE / (k_ms / 1e3) / 1e12
+    task.close()
+
+    # ---- plain kernel on a prefix of the same positions --------------------------------
+    pn = min(args.plain_n, n)
+    plain_cfg = go2.default_config(ordering=go2.ORDER_STATIC)
+    pt = go2.Task(pn, E, plain_cfg)
+    pt.upload(boards[:pn])
+    pt.launch()
+    pt.wait()
+    plain_scores, pst = pt.download()
+    pt.close()
+    plain_value = pn / (pst["kernel_ms"] / 1e3)
+    # the headline solver on the same prefix, same batch size
+    ht = go2.Task(pn, E, cfg)
+    ht.upload(boards[:pn])
+    ht.launch()
+    ht.wait()
+    head_scores, hst = ht.download()
+    ht.close()
+    head_small_value = pn / (hst["kernel_ms"] / 1e3)
+    plain_match = bool(np.array_equal(plain_scores, head_scores) and np.array_equal(head_scores, scores[:pn]))
+    log(f"[bench] plain on {pn}: {plain_value:.0f} pos/s; headline on the same {pn}: {head_small_value:.0f} pos/s")
+
+    # ---- CPU baseline: the reference method on the host cores ----------------------------
+    cpu = None
+    cpu_match = None
+    if not args.skip_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as ol
+
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        k = min(args.cpu_sample, n)
+        b2 = np.stack([boards["player"][:k], boards["opponent"][:k]], 1)
+        cs, cnodes, secs = ol.solve_batch(b2, E, threads=threads)
+        cpu_match = bool(np.array_equal(cs, scores[:k]))
+        cpu = {"value": k / secs, "unit": "positions/s", "cores": threads, "kind": "port",
+               "sample": f"first {k} of the same {E}-empty positions, reference method "
+                         f"(fastest-first in the top DEPTH-9 plies, corner/LSB order below, no table), "
+                         f"{threads} threads, {secs:.1f} s"}
+        log(f"[bench] cpu baseline {k / secs:.1f} pos/s on {threads} threads; match={cpu_match}")
+
+    line = {
+        "metric": "exact-solved positions/sec @16 empties, 1 GPU (learned-order vs plain), 100% match",
+        "value": value,
+        "unit": "positions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"exact solve of {n} random-playout positions at {E} empties",
+            "positions": n, "empties": E, "ordering": order_name,
+            "plain_positions": pn, "plain_value": plain_value,
+            "headline_value_same_sample": head_small_value,
+            "ratio_vs_plain_same_sample": head_small_value / plain_value,
+            "score_match_plain": plain_match, "score_match_cpu": cpu_match,
+            "nodes_per_position": nodes_per_launch / n,
+        },
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_INT32_TOPS,
+                     "unit": "Tops/s (int32 lane-ops)", "frac": achieved / PEAK_INT32_TOPS,
+                     "traffic": None, "kernel_ms": k_ms,
+                     "ops_per_node": ALGO_INT_OPS_PER_NODE},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0 if plain_match and cpu_match is not False else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

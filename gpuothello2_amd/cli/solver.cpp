@@ -1,0 +1,166 @@
+// solver INPUT OUTPUT DEPTH — drop-in for the reference's solve mode
+// (main.cu:123-185): reads n base81 boards, sorts and de-duplicates them,
+// solves every board exactly on the GPU through libgo2, and writes one
+// "<base81> <score>" line per unique board in sorted (player, opponent) order.
+//
+// Behaviour kept from the reference:
+//   * input: first line n, then n lines whose first 16 bytes are a base81 board
+//     (main.cu:49-63); a missing line prints "Failed to load boards" and exits 1;
+//   * boards sorted and de-duplicated (main.cu:64-65);
+//   * batches of at most 2,000,000 boards, each on its own stream, all launched
+//     before polling (main.cu:130-155);
+//   * stderr lines "n = ...", "start!", the status/elapsed/table line and
+//     "total nodes: ..." (main.cu:129,144,158-162,172);
+//   * usage text and exit code 1 for fewer than three arguments (main.cu:176-179).
+// Undefined behaviour of the reference becomes a defined error (exit code 2):
+// unreadable files, a non-numeric or out-of-range DEPTH, a board with more
+// empties than DEPTH, malformed characters, overlapping discs, device errors.
+// Think mode (five arguments, main.cu:180-183) is not part of this solver and
+// is rejected with exit code 2.
+//
+// Search knobs come from the environment (GO2_ORDERING=static|mobility|learned,
+// GO2_BLOCK_SIZE, GO2_MOBILITY_MIN_EMPTIES, GO2_LEARNED_MIN_EMPTIES); the
+// defaults are go2_config_default()'s.
+#include <go2.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cinttypes>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace {
+
+struct FileCloser {
+  void operator()(FILE* f) const {
+    if (f) fclose(f);
+  }
+};
+using File = std::unique_ptr<FILE, FileCloser>;
+
+constexpr size_t kBatch = 2000000;
+
+[[noreturn]] void die(int code, const std::string& msg) {
+  fprintf(stderr, "error: %s\n", msg.c_str());
+  exit(code);
+}
+
+bool board_less(const go2_board& a, const go2_board& b) {
+  return a.player != b.player ? a.player < b.player : a.opponent < b.opponent;
+}
+bool board_eq(const go2_board& a, const go2_board& b) {
+  return a.player == b.player && a.opponent == b.opponent;
+}
+
+std::vector<go2_board> load_boards(const char* path) {
+  File in(fopen(path, "r"));
+  if (!in) die(2, std::string("cannot open input ") + path);
+  long long n = -1;
+  if (fscanf(in.get(), "%lld\n", &n) != 1 || n < 0) die(2, "input does not start with a board count");
+  std::vector<go2_board> v((size_t)n);
+  for (long long i = 0; i < n; ++i) {
+    char buf[33] = {0};
+    if (!fgets(buf, 32, in.get())) {
+      fprintf(stderr, "Failed to load boards\n");
+      exit(EXIT_FAILURE);
+    }
+    if (strnlen(buf, 32) < 16) die(2, "line " + std::to_string(i + 2) + " is shorter than 16 characters");
+    if (go2_board_from_text(buf, &v[(size_t)i]) != GO2_OK)
+      die(2, "line " + std::to_string(i + 2) + ": " + go2_last_error());
+    if (v[(size_t)i].player & v[(size_t)i].opponent)
+      die(2, "line " + std::to_string(i + 2) + ": a square holds both colours");
+  }
+  std::sort(v.begin(), v.end(), board_less);
+  v.erase(std::unique(v.begin(), v.end(), board_eq), v.end());
+  return v;
+}
+
+int env_int(const char* name, int def) {
+  const char* s = getenv(name);
+  return s && *s ? atoi(s) : def;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 4) {
+    fprintf(stderr, "usage: %s INPUT OUTPUT DEPTH [THINK_DEPTH]\n", argv[0]);
+    return 1;
+  }
+  if (argc >= 5) die(2, "think mode (THINK_DEPTH) is not provided by this solver");
+
+  char* end = nullptr;
+  const long depth = strtol(argv[3], &end, 10);
+  if (!end || *end || depth < 1 || depth > GO2_MAX_EMPTIES)
+    die(2, "DEPTH must be an integer in [1, " + std::to_string(GO2_MAX_EMPTIES) + "]");
+
+  const auto boards = load_boards(argv[1]);
+  for (size_t i = 0; i < boards.size(); ++i) {
+    const int e = 64 - __builtin_popcountll(boards[i].player | boards[i].opponent);
+    if (e > depth) die(2, "a board has " + std::to_string(e) + " empties, more than DEPTH");
+  }
+  File out(fopen(argv[2], "w"));
+  if (!out) die(2, std::string("cannot open output ") + argv[2]);
+
+  go2_config cfg;
+  go2_config_default(&cfg);
+  if (const char* o = getenv("GO2_ORDERING")) {
+    if (!strcmp(o, "static")) cfg.ordering = GO2_ORDER_STATIC;
+    else if (!strcmp(o, "mobility")) cfg.ordering = GO2_ORDER_MOBILITY;
+    else if (!strcmp(o, "learned")) cfg.ordering = GO2_ORDER_LEARNED;
+    else die(2, "GO2_ORDERING must be static, mobility or learned");
+  }
+  cfg.block_size = env_int("GO2_BLOCK_SIZE", cfg.block_size);
+  cfg.mobility_min_empties = env_int("GO2_MOBILITY_MIN_EMPTIES", cfg.mobility_min_empties);
+  cfg.learned_min_empties = env_int("GO2_LEARNED_MIN_EMPTIES", cfg.learned_min_empties);
+
+  const size_t n = boards.size();
+  fprintf(stderr, "n = %zu\n", n);
+  std::vector<go2_task*> tasks;
+  for (size_t i = 0; i < n; i += kBatch) {
+    const size_t cnt = std::min(kBatch, n - i);
+    go2_task* t = go2_task_create(cnt, (int)depth, &cfg);
+    if (!t) die(2, go2_last_error());
+    if (go2_task_upload(t, boards.data() + i, cnt) != GO2_OK) die(2, go2_last_error());
+    tasks.push_back(t);
+  }
+  fprintf(stderr, "start!\n");
+  const auto t0 = std::chrono::steady_clock::now();
+  for (go2_task* t : tasks)
+    if (go2_task_launch(t) != GO2_OK) die(2, go2_last_error());
+  for (;;) {
+    bool done = true;
+    for (go2_task* t : tasks) {
+      const int r = go2_task_query(t);
+      if (r < 0) die(2, go2_last_error());
+      if (r == 0) done = false;
+    }
+    if (done) break;
+    std::this_thread::sleep_for(std::chrono::microseconds(200));
+  }
+  const auto t1 = std::chrono::steady_clock::now();
+  std::vector<int8_t> scores(n);
+  uint64_t nodes = 0;
+  for (size_t k = 0; k < tasks.size(); ++k) {
+    go2_stats st;
+    if (go2_task_download(tasks[k], scores.data() + k * kBatch, &st) != GO2_OK) die(2, go2_last_error());
+    nodes += st.nodes;
+    go2_task_destroy(tasks[k]);
+  }
+  const long ms = (long)std::chrono::duration_cast<std::chrono::milliseconds>(t1 - t0).count();
+  fprintf(stderr,
+          "hipSuccess (no error), elapsed: %ldms, table update count: %d, table hit: %d, table find: %d\n",
+          ms, 0, 0, 0);
+  char buf[17];
+  for (size_t j = 0; j < n; ++j) {
+    go2_board_to_text(&boards[j], buf);
+    fprintf(out.get(), "%s %d\n", buf, (int)scores[j]);
+  }
+  fprintf(stderr, "total nodes: %" PRIu64 "\n", nodes);
+  return 0;
+}

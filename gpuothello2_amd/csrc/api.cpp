@@ -1,0 +1,321 @@
+// C ABI of libgo2: argument checks, device buffers, streams, events and the
+// base81 board text.  Kernels live in solver.hip / generator.hip; this file
+// only orchestrates them.
+//
+// Counterparts in the reference:
+//   go2_solve / go2_solve_device  <- solve()          main.cu:123-173 (minus file I/O)
+//   go2_task_*                    <- BatchedTask      solver.cuh:8-25, solver.cu:313-348
+//   go2_board_from_text/_to_text  <- toBoard/fromBoard to_board.cpp:15-60
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "internal.hpp"
+
+namespace go2 {
+
+static thread_local std::string g_last_error;
+static thread_local int g_last_code = GO2_OK;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  g_last_code = code;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(GO2_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int resolve_config(const go2_config* cfg, go2_config* out) {
+  go2_config c;
+  if (cfg) {
+    c = *cfg;
+  } else {
+    go2_config_default(&c);
+  }
+  if (c.ordering < GO2_ORDER_STATIC || c.ordering > GO2_ORDER_LEARNED)
+    return fail(GO2_E_ARG, "config.ordering must be GO2_ORDER_STATIC, _MOBILITY or _LEARNED");
+  if (c.block_size < 64 || c.block_size > 1024 || c.block_size % 64)
+    return fail(GO2_E_ARG, "config.block_size must be a multiple of 64 in [64, 1024]");
+  if (c.mobility_min_empties < 3) c.mobility_min_empties = 3;
+  if (c.learned_min_empties < 3) c.learned_min_empties = 3;
+  if (c.blocks_per_cu < 0) return fail(GO2_E_ARG, "config.blocks_per_cu must be >= 0");
+  *out = c;
+  return GO2_OK;
+}
+
+// Device buffers of one solve: counters plus, for host-array calls, the
+// boards and scores.
+struct DeviceBufs {
+  SolveCounters* ctr = nullptr;
+  go2_board* boards = nullptr;
+  int8_t* scores = nullptr;
+  size_t cap = 0;
+  ~DeviceBufs() {
+    if (ctr) (void)hipFree(ctr);
+    if (boards) (void)hipFree(boards);
+    if (scores) (void)hipFree(scores);
+  }
+  int alloc(size_t n, bool io) {
+    hipError_t e;
+    if (!ctr && (e = hipMalloc(&ctr, sizeof(SolveCounters))) != hipSuccess)
+      return hip_fail(e, "hipMalloc(counters)");
+    if (io && n > cap) {
+      if (boards) (void)hipFree(boards);
+      if (scores) (void)hipFree(scores);
+      boards = nullptr;
+      scores = nullptr;
+      cap = 0;
+      if ((e = hipMalloc(&boards, n * sizeof(go2_board))) != hipSuccess)
+        return hip_fail(e, "hipMalloc(boards)");
+      if ((e = hipMalloc(&scores, n)) != hipSuccess) return hip_fail(e, "hipMalloc(scores)");
+      cap = n;
+    }
+    return GO2_OK;
+  }
+};
+
+static int check_depth(int max_depth) {
+  if (max_depth < 1 || max_depth > GO2_MAX_EMPTIES)
+    return fail(GO2_E_ARG, "max_depth must be in [1, " + std::to_string(GO2_MAX_EMPTIES) + "]");
+  return GO2_OK;
+}
+
+struct Events {
+  hipEvent_t a = nullptr, b = nullptr;
+  ~Events() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+  int create() {
+    hipError_t e;
+    if ((e = hipEventCreate(&a)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+    if ((e = hipEventCreate(&b)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+    return GO2_OK;
+  }
+};
+
+}  // namespace go2
+
+using namespace go2;
+
+struct go2_task {
+  size_t n = 0;
+  size_t count = 0;
+  int max_depth = 0;
+  go2_config cfg{};
+  hipStream_t stream = nullptr;
+  DeviceBufs bufs;
+  Events ev;
+  bool launched = false;
+};
+
+extern "C" {
+
+int go2_version(void) { return 1; }
+
+const char* go2_last_error(void) { return g_last_error.c_str(); }
+
+int go2_set_device(int device) {
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  return GO2_OK;
+}
+
+void go2_config_default(go2_config* cfg) {
+  if (!cfg) return;
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->ordering = GO2_ORDER_MOBILITY;
+  cfg->mobility_min_empties = 7;
+  cfg->learned_min_empties = 12;
+  cfg->block_size = 256;
+  cfg->blocks_per_cu = 0;
+}
+
+// base81: character i encodes squares 4i..4i+3 as 33 + d0 + 3 d1 + 9 d2 + 32 d3,
+// digit 0 empty / 1 side to move / 2 other side (reference test.cpp:7-19).
+int go2_board_from_text(const char* text, go2_board* out) {
+  if (!text || !out) return fail(GO2_E_ARG, "null argument");
+  uint64_t p = 0, o = 0;
+  for (int i = 0; i < 16; ++i) {
+    const int x = (unsigned char)text[i] - 33;
+    const int hi = x >> 5, lo = x & 31;
+    if (x < 0 || hi > 2 || lo > 26)
+      return fail(GO2_E_PARSE, "character " + std::to_string(i) + " of board text is not base81");
+    const int d[4] = {lo % 3, (lo / 3) % 3, lo / 9, hi};
+    for (int j = 0; j < 4; ++j) {
+      const uint64_t b = 1ULL << (4 * i + j);
+      if (d[j] == 1) p |= b;
+      if (d[j] == 2) o |= b;
+    }
+  }
+  out->player = p;
+  out->opponent = o;
+  return GO2_OK;
+}
+
+void go2_board_to_text(const go2_board* board, char out[17]) {
+  static const int w[4] = {1, 3, 9, 32};
+  for (int i = 0; i < 16; ++i) {
+    int v = 33;
+    for (int j = 0; j < 4; ++j) {
+      const int sq = 4 * i + j;
+      v += (int)((board->player >> sq) & 1) * w[j] + 2 * (int)((board->opponent >> sq) & 1) * w[j];
+    }
+    out[i] = (char)v;
+  }
+  out[16] = '\0';
+}
+
+int go2_solve_device(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_scores,
+                     const go2_config* cfg, go2_stats* stats, void* hip_stream) {
+  int rc;
+  if ((rc = check_depth(max_depth)) != GO2_OK) return rc;
+  if (n && (!d_boards || !d_scores)) return fail(GO2_E_ARG, "null device array");
+  go2_config c;
+  if ((rc = resolve_config(cfg, &c)) != GO2_OK) return rc;
+  hipStream_t s = (hipStream_t)hip_stream;
+  DeviceBufs bufs;
+  Events ev;
+  if ((rc = bufs.alloc(0, false)) != GO2_OK) return rc;
+  if ((rc = ev.create()) != GO2_OK) return rc;
+  if (n == 0) {
+    if (stats) *stats = go2_stats{};
+    return GO2_OK;
+  }
+  if ((rc = launch_solve(d_boards, n, max_depth, d_scores, c, bufs.ctr, s, ev.a, ev.b)) != GO2_OK)
+    return rc;
+  return collect_stats(bufs.ctr, n, s, ev.a, ev.b, stats);
+}
+
+int go2_solve(const go2_board* boards, size_t n, int max_depth, int8_t* scores,
+              const go2_config* cfg, go2_stats* stats) {
+  if (n && (!boards || !scores)) return fail(GO2_E_ARG, "null host array");
+  go2_task* t = go2_task_create(n, max_depth, cfg);
+  if (!t) return g_last_code;
+  int rc = go2_task_upload(t, boards, n);
+  if (rc == GO2_OK) rc = go2_task_launch(t);
+  if (rc == GO2_OK) rc = go2_task_download(t, scores, stats);
+  go2_task_destroy(t);
+  return rc;
+}
+
+go2_task* go2_task_create(size_t n, int max_depth, const go2_config* cfg) {
+  if (check_depth(max_depth) != GO2_OK) return nullptr;
+  go2_task* t = new go2_task;
+  t->n = n;
+  t->max_depth = max_depth;
+  if (resolve_config(cfg, &t->cfg) != GO2_OK) {
+    delete t;
+    return nullptr;
+  }
+  hipError_t e = hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    hip_fail(e, "hipStreamCreate");
+    delete t;
+    return nullptr;
+  }
+  if (t->bufs.alloc(n ? n : 1, true) != GO2_OK || t->ev.create() != GO2_OK) {
+    (void)hipStreamDestroy(t->stream);
+    delete t;
+    return nullptr;
+  }
+  return t;
+}
+
+int go2_task_upload(go2_task* t, const go2_board* boards, size_t count) {
+  if (!t) return fail(GO2_E_ARG, "null task");
+  if (count > t->n) return fail(GO2_E_ARG, "upload count exceeds the task size");
+  if (count && !boards) return fail(GO2_E_ARG, "null boards");
+  if (count) {
+    hipError_t e = hipMemcpyAsync(t->bufs.boards, boards, count * sizeof(go2_board),
+                                  hipMemcpyHostToDevice, t->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(boards)");
+    if ((e = hipStreamSynchronize(t->stream)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+  }
+  t->count = count;
+  t->launched = false;
+  return GO2_OK;
+}
+
+int go2_task_launch(go2_task* t) {
+  if (!t) return fail(GO2_E_ARG, "null task");
+  int rc = GO2_OK;
+  if (t->count)
+    rc = launch_solve(t->bufs.boards, t->count, t->max_depth, t->bufs.scores, t->cfg, t->bufs.ctr,
+                      t->stream, t->ev.a, t->ev.b);
+  if (rc == GO2_OK) t->launched = true;
+  return rc;
+}
+
+int go2_task_query(go2_task* t) {
+  if (!t) return fail(GO2_E_ARG, "null task");
+  if (!t->launched) return fail(GO2_E_STATE, "task not launched");
+  hipError_t e = hipStreamQuery(t->stream);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  return hip_fail(e, "hipStreamQuery");
+}
+
+int go2_task_wait(go2_task* t) {
+  if (!t) return fail(GO2_E_ARG, "null task");
+  if (!t->launched) return fail(GO2_E_STATE, "task not launched");
+  hipError_t e = hipStreamSynchronize(t->stream);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+  return GO2_OK;
+}
+
+int go2_task_download(go2_task* t, int8_t* scores, go2_stats* stats) {
+  if (!t) return fail(GO2_E_ARG, "null task");
+  if (!t->launched) return fail(GO2_E_STATE, "task not launched");
+  if (t->count && !scores) return fail(GO2_E_ARG, "null scores");
+  if (t->count == 0) {
+    if (stats) *stats = go2_stats{};
+    return GO2_OK;
+  }
+  int rc = collect_stats(t->bufs.ctr, t->count, t->stream, t->ev.a, t->ev.b, stats);
+  if (rc != GO2_OK) return rc;
+  hipError_t e = hipMemcpyAsync(scores, t->bufs.scores, t->count, hipMemcpyDeviceToHost, t->stream);
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(scores)");
+  if ((e = hipStreamSynchronize(t->stream)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+  return GO2_OK;
+}
+
+void go2_task_destroy(go2_task* t) {
+  if (!t) return;
+  if (t->stream) {
+    (void)hipStreamSynchronize(t->stream);
+    (void)hipStreamDestroy(t->stream);
+  }
+  delete t;
+}
+
+int go2_generate_device(uint64_t seed, uint32_t config_id, size_t n, int empties_min,
+                        int empties_max, go2_board* d_out, void* hip_stream) {
+  if (empties_min < 0 || empties_max > 60 || empties_min > empties_max)
+    return fail(GO2_E_ARG, "need 0 <= empties_min <= empties_max <= 60");
+  if (n && !d_out) return fail(GO2_E_ARG, "null output");
+  return launch_generate(seed, config_id, n, empties_min, empties_max, d_out,
+                         (hipStream_t)hip_stream);
+}
+
+int go2_generate(uint64_t seed, uint32_t config_id, size_t n, int empties_min, int empties_max,
+                 go2_board* out) {
+  if (n && !out) return fail(GO2_E_ARG, "null output");
+  if (n == 0) return GO2_OK;
+  go2_board* d = nullptr;
+  hipError_t e = hipMalloc(&d, n * sizeof(go2_board));
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc(generate)");
+  int rc = go2_generate_device(seed, config_id, n, empties_min, empties_max, d, nullptr);
+  if (rc == GO2_OK) {
+    e = hipMemcpy(out, d, n * sizeof(go2_board), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = hip_fail(e, "hipMemcpy(generate)");
+  }
+  (void)hipFree(d);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,139 @@
+/*
+ * go2 — C ABI of the MI355X (gfx950) batched exact Othello endgame solver.
+ *
+ * Plain C types only (no torch, no HIP types in signatures).  Every call
+ * returns GO2_OK (0) or a negative GO2_E* code; go2_last_error() gives the
+ * message of the calling thread's last failure.  The library owns every
+ * device buffer it allocates; callers own their host arrays.  A task handle
+ * is used by one host thread at a time; many handles may run concurrently,
+ * each on its own HIP stream.
+ *
+ * Board layout (all entry points): square = row*8 + col, bit 0 = A1,
+ * bit 7 = H1, bit 63 = H8; `player` is the side to move.
+ * Scores are exact disc differences in [-64, 64] from the side to move,
+ * winner taking the empty squares (reference board.cuh:7-18).
+ */
+#ifndef GO2_H
+#define GO2_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GO2_OK 0
+#define GO2_E_ARG -1     /* bad argument (null pointer, size, depth) */
+#define GO2_E_BOARD -2   /* invalid board: overlapping discs, or more empties than DEPTH */
+#define GO2_E_DEVICE -3  /* HIP runtime failure */
+#define GO2_E_NOMEM -4   /* allocation failure */
+#define GO2_E_STATE -5   /* call out of order (e.g. download before launch) */
+#define GO2_E_PARSE -6   /* malformed base81 text */
+
+/* Largest number of empty squares one board may have. */
+#define GO2_MAX_EMPTIES 32
+
+/* Move ordering of the solver. */
+#define GO2_ORDER_STATIC 0   /* corners, then plain squares, X-squares last */
+#define GO2_ORDER_MOBILITY 1 /* fastest-first (fewest replies) above a cutoff, parity below */
+#define GO2_ORDER_LEARNED 2  /* as MOBILITY, plus the pattern evaluator above a higher cutoff */
+
+/* One input position.  Replaces AlphaBetaProblem (reference alphabeta.cuh:12-21);
+ * the window is always the full (-64, 64) one the reference's CLI passes
+ * (reference main.cu:140), so it is not part of the record. */
+typedef struct go2_board {
+  uint64_t player;
+  uint64_t opponent;
+} go2_board;
+
+/* Launch and search knobs.  go2_config_default() fills a valid set. */
+typedef struct go2_config {
+  int32_t ordering;            /* GO2_ORDER_* */
+  int32_t mobility_min_empties; /* fastest-first at nodes with >= this many empties */
+  int32_t learned_min_empties;  /* evaluator ordering at nodes with >= this many empties */
+  int32_t block_size;          /* lanes per workgroup, multiple of 64, 64..1024 */
+  int32_t blocks_per_cu;       /* resident workgroups per CU to launch; 0 = as many as LDS allows */
+  int32_t reserved[11];
+} go2_config;
+
+/* Counters of one solve.  Replaces the reference's `total` node counter
+ * (reference solver.cu:308-309) with defined meanings. */
+typedef struct go2_stats {
+  uint64_t positions;   /* boards solved */
+  uint64_t nodes;       /* positions visited: roots, pushed children, leaves scored in place */
+  uint64_t score_steps; /* child evaluations spent only on move ordering */
+  uint64_t iterations;  /* lane-iterations of the search loop (sum over lanes) */
+  double kernel_ms;     /* device time of the solve kernel(s), HIP events on the solve stream */
+  uint64_t reserved[8];
+} go2_stats;
+
+/* ---- library ---------------------------------------------------------------- */
+int go2_version(void);
+const char *go2_last_error(void);
+/* Select the HIP device for the calling thread's subsequent calls. */
+int go2_set_device(int device);
+void go2_config_default(go2_config *cfg);
+
+/* ---- board text (replaces toBoard/fromBoard, reference to_board.hpp:6-9,
+ *      to_board.cpp:15-60) ------------------------------------------------------- */
+/* Decode the first 16 characters of `text`.  GO2_E_PARSE on a character that
+ * is no base81 digit group (the reference decodes garbage silently). */
+int go2_board_from_text(const char *text, go2_board *out);
+/* Encode to 16 characters plus a terminating NUL. */
+void go2_board_to_text(const go2_board *board, char out[17]);
+
+/* ---- one-shot solve ------------------------------------------------------------ */
+/* Host arrays in, host scores out; blocks until done.  max_depth is the
+ * reference's DEPTH (maximum number of empties, reference README.md:48-50,
+ * main.cu:127): every board must have at most max_depth empties, and
+ * max_depth <= GO2_MAX_EMPTIES.  Replaces the solve path of reference
+ * main.cu:123-173 minus file I/O. */
+int go2_solve(const go2_board *boards, size_t n, int max_depth, int8_t *scores,
+              const go2_config *cfg, go2_stats *stats);
+
+/* Device arrays in and out (inputs already resident in HBM).  Enqueued on
+ * `hip_stream` (a hipStream_t, or NULL for the library's own stream) and
+ * waited for before returning, so `stats` is complete on return. */
+int go2_solve_device(const go2_board *d_boards, size_t n, int max_depth, int8_t *d_scores,
+                     const go2_config *cfg, go2_stats *stats, void *hip_stream);
+
+/* ---- batched task (replaces BatchedTask, reference solver.cuh:8-25,
+ *      solver.cu:313-348) ------------------------------------------------------------ */
+typedef struct go2_task go2_task;
+/* Allocate device buffers for n boards and a private stream (reference
+ * solver.cu:313-323).  NULL on failure (see go2_last_error). */
+go2_task *go2_task_create(size_t n, int max_depth, const go2_config *cfg);
+/* Copy boards [0, count) into the task (the reference's `abp[j] = ...`,
+ * main.cu:138-141).  count <= n. */
+int go2_task_upload(go2_task *task, const go2_board *boards, size_t count);
+/* Enqueue the solve asynchronously (reference solver.cu:330-333). */
+int go2_task_launch(go2_task *task);
+/* 1 when finished, 0 while running, negative on error (reference
+ * solver.cu:335-337 — which reports an error as "ready"; this one does not). */
+int go2_task_query(go2_task *task);
+/* Block until finished. */
+int go2_task_wait(go2_task *task);
+/* Copy scores (count entries) and stats out (the reference's result[] and
+ * *total, main.cu:165-171).  Waits for the task first. */
+int go2_task_download(go2_task *task, int8_t *scores, go2_stats *stats);
+/* Free device buffers and the stream (reference solver.cu:339-348). */
+void go2_task_destroy(go2_task *task);
+
+/* ---- synthetic positions -------------------------------------------------------- */
+/* Random legal playouts from the standard start (passing when forced) to a
+ * fixed number of empties; a game that ends first is replayed with the next
+ * attempt number.  Position i uses the seed-derived stream (seed, config_id,
+ * i, attempt); its empties are drawn uniformly from [empties_min,
+ * empties_max].  Runs on the GPU; writes n boards to host memory `out`.
+ * No counterpart in the reference (its inputs come from files). */
+int go2_generate(uint64_t seed, uint32_t config_id, size_t n, int empties_min, int empties_max,
+                 go2_board *out);
+/* Same, writing device memory. */
+int go2_generate_device(uint64_t seed, uint32_t config_id, size_t n, int empties_min,
+                        int empties_max, go2_board *d_out, void *hip_stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GO2_H */

@@ -1,0 +1,164 @@
+"""Parity of the HIP solver with the oracle, through the C ABI.  Every score
+must be bit-exact."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import gpuothello2_amd as go2
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+ORDERINGS = {"static": go2.ORDER_STATIC, "mobility": go2.ORDER_MOBILITY}
+
+
+def cfg(name, **kw):
+    return go2.default_config(ordering=ORDERINGS[name], **kw)
+
+
+def sym(b: np.ndarray, s: int) -> np.ndarray:
+    """Image of (n,2) boards under symmetry s of the square (vectorised)."""
+    out = np.zeros_like(b)
+    for sq in range(64):
+        r, c = sq >> 3, sq & 7
+        if s & 4:
+            r, c = c, r
+        if s & 1:
+            c = 7 - c
+        if s & 2:
+            r = 7 - r
+        t = r * 8 + c
+        bit = (b >> np.uint64(sq)) & np.uint64(1)
+        out |= bit << np.uint64(t)
+    return out
+
+
+@pytest.mark.parametrize("ordering", ["static", "mobility"])
+def test_small_fixture(ordering):
+    d = np.load(os.path.join(GOLD, "solve_small.npz"))
+    s, st = go2.solve(go2.as_boards(d["player"], d["opponent"]), 10, cfg(ordering))
+    assert np.array_equal(s, d["score"])
+    assert st["positions"] == len(s)
+
+
+@pytest.mark.parametrize("ordering", ["static", "mobility"])
+def test_mid_fixture(ordering):
+    d = np.load(os.path.join(GOLD, "solve_mid.npz"))
+    s, _ = go2.solve(go2.as_boards(d["player"], d["opponent"]), 14, cfg(ordering))
+    assert np.array_equal(s, d["score"])
+
+
+def test_readme_boards():
+    rows = json.load(open(os.path.join(GOLD, "readme_boards.json")))
+    b = go2.as_boards([int(r["player"]) for r in rows], [int(r["opponent"]) for r in rows])
+    for depth in (10, 12):
+        s, _ = go2.solve(b, depth)
+        assert list(s) == [r["score"] for r in rows]
+
+
+@pytest.mark.parametrize("empties,n", [(6, 3000), (10, 3000), (12, 1500)])
+def test_random_vs_oracle(empties, n):
+    b = ol.generate(2026, 7, n, empties)
+    want, _, _ = ol.solve_batch(b, max(empties, 10), threads=16)
+    for name in ORDERINGS:
+        got, st = go2.solve(go2.as_boards(b), empties, cfg(name))
+        assert np.array_equal(got, want), name
+        assert st["nodes"] >= n
+
+
+def test_uneven_depths_vs_oracle():
+    rows = [ol.generate(5, 3, 1, int(ol.empties_for(5, 3, i, 0, 13)), start=i)[0] for i in range(1200)]
+    b = np.array(rows, dtype=np.uint64)
+    want, _, _ = ol.solve_batch(b, 13, threads=16)
+    got, _ = go2.solve(go2.as_boards(b), 13)
+    assert np.array_equal(got, want)
+
+
+def test_generator_matches_oracle():
+    n = 2000
+    g = go2.generate(99, 3, n, 8, 20)
+    for i in range(0, n, 7):
+        e = ol.empties_for(99, 3, i, 8, 20)
+        want = ol.generate(99, 3, 1, e, start=i)[0]
+        assert (int(g["player"][i]), int(g["opponent"][i])) == (int(want[0]), int(want[1])), i
+    e = 64 - np.array([bin(int(p) | int(o)).count("1") for p, o in zip(g["player"], g["opponent"])])
+    assert e.min() >= 8 and e.max() <= 20
+
+
+def test_symmetry_invariance_large():
+    # Size-independent property at a larger size: scores are invariant under
+    # the 8 symmetries of the board.
+    g = go2.generate(4242, 2, 20000, 12)
+    b = np.stack([g["player"], g["opponent"]], 1)
+    base, _ = go2.solve(go2.as_boards(b), 12)
+    for s in (1, 2, 5, 7):
+        img, _ = go2.solve(go2.as_boards(sym(b, s)), 12, cfg("static"))
+        assert np.array_equal(img, base), s
+
+
+def test_task_api_roundtrip():
+    b = ol.generate(31, 1, 500, 11)
+    want, _, _ = ol.solve_batch(b, 11, threads=16)
+    t = go2.Task(1000, 11)
+    t.upload(go2.as_boards(b))
+    t.launch()
+    t.wait()
+    assert t.is_ready()
+    s, st = t.download()
+    assert np.array_equal(s, want)
+    assert st["positions"] == 500 and st["kernel_ms"] > 0
+    t.close()
+
+
+def test_device_entry_point():
+    from gpuothello2_amd.hip import DeviceBuffer, Stream
+    b = ol.generate(8, 8, 700, 12)
+    want, _, _ = ol.solve_batch(b, 12, threads=16)
+    d_b = DeviceBuffer(b.nbytes)
+    d_s = DeviceBuffer(700)
+    d_b.upload(np.ascontiguousarray(b))
+    s = Stream()
+    st = go2.solve_device(d_b.addr, 700, 12, d_s.addr, go2.default_config(), s.handle.value)
+    got = np.zeros(700, dtype=np.int8)
+    d_s.download(got)
+    assert np.array_equal(got, want)
+    assert st["positions"] == 700 and st["kernel_ms"] > 0
+    st = go2.solve_device(d_b.addr, 700, 12, d_s.addr, go2.default_config(ordering=go2.ORDER_STATIC), None)
+    d_s.download(got)
+    assert np.array_equal(got, want)
+    s.destroy()
+
+
+def test_invalid_boards_are_rejected():
+    bad = go2.as_boards([0xFF, 0x0000000810000000], [0x1, 0x0000001008000000])
+    with pytest.raises(go2.Go2Error) as e:
+        go2.solve(bad, 12)
+    assert e.value.code == go2.GO2_E_BOARD
+    deep = ol.generate(3, 3, 4, 14)
+    with pytest.raises(go2.Go2Error) as e:
+        go2.solve(go2.as_boards(deep), 12)
+    assert e.value.code == go2.GO2_E_BOARD
+
+
+def test_empty_batch():
+    s, st = go2.solve(go2.as_boards(np.zeros((0, 2), dtype=np.uint64)), 12)
+    assert s.shape == (0,) and st["positions"] == 0
+
+
+def test_cli_matches_oracle(tmp_path):
+    import subprocess
+    b = ol.generate(17, 1, 300, 10)
+    b = np.concatenate([b, b[:50]])  # duplicates are removed
+    lines = [ol.encode(int(p), int(o)) for p, o in b]
+    inp = tmp_path / "in.txt"
+    inp.write_text(f"{len(lines)}\n" + "\n".join(lines) + "\n")
+    out = tmp_path / "out.txt"
+    r = subprocess.run([go2.SOLVER_BIN, str(inp), str(out), "10"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "n = 300" in r.stderr and "total nodes:" in r.stderr
+    uniq = sorted({(int(p), int(o)) for p, o in b})
+    want, _, _ = ol.solve_batch(np.array(uniq, dtype=np.uint64), 10, threads=16)
+    expect = "".join(f"{ol.encode(p, o)} {int(s)}\n" for (p, o), s in zip(uniq, want))
+    assert out.read_text() == expect

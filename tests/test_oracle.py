@@ -1,0 +1,188 @@
+"""Pins the CPU oracle: against the reference's own host functions (golden
+vectors written from them), against brute force, and against what the game's
+mathematics fixes.  CPU only."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+MASK = (1 << 64) - 1
+
+
+@pytest.fixture(scope="module")
+def O():
+    return ol.oracle()
+
+
+@pytest.fixture(scope="module")
+def refvec():
+    return np.load(os.path.join(GOLD, "ref_bitboard.npz"))
+
+
+def test_flip_matches_reference_vectors(O, refvec):
+    P, Op, F = refvec["player"], refvec["opponent"], refvec["flip"]
+    bad = 0
+    for i in range(P.shape[0]):
+        p, o = int(P[i]), int(Op[i])
+        for sq in range(64):
+            if ((p | o) >> sq) & 1:
+                continue
+            want = int(F[i, sq])
+            bad += O.orc_flip(p, o, sq) != want
+            bad += O.orc_flip_walk(p, o, sq) != want
+    assert bad == 0
+
+
+def test_mobility_and_score_match_reference_vectors(O, refvec):
+    for p, o, m, s in zip(refvec["player"], refvec["opponent"], refvec["mobility"], refvec["final_score"]):
+        p, o = int(p), int(o)
+        assert O.orc_mobility(p, o) == int(m)
+        assert O.orc_mobility_walk(p, o) == int(m)
+        assert O.orc_final_score(p, o) == int(s)
+
+
+def test_base81_matches_reference_vectors(refvec):
+    for p, o, t in zip(refvec["player"], refvec["opponent"], refvec["text"]):
+        assert ol.encode(int(p), int(o)) == str(t)
+        assert ol.decode(str(t)) == (int(p), int(o))
+
+
+def test_base81_closed_form():
+    # test.cpp:7-19 of the reference: char i = 33 + sum_j w_j * digit(4i+j), w = 1,3,9,32.
+    rng = np.random.default_rng(3)
+    w = [1, 3, 9, 32]
+    for _ in range(300):
+        x, y = (int(v) for v in rng.integers(0, 2**63, 2)), None
+        x = int(rng.integers(0, 2**63)) << 1 | 1
+        y = int(rng.integers(0, 2**63)) << 1
+        p, o = x & y & MASK, x & ~y & MASK
+        s = "".join(chr(33 + sum(w[j] * (((p >> (4 * i + j)) & 1) + 2 * ((o >> (4 * i + j)) & 1))
+                                 for j in range(4))) for i in range(16))
+        assert ol.encode(p, o) == s
+        assert ol.decode(s) == (p, o)
+
+
+def test_base81_rejects_bad_chars():
+    assert ol.decode(" " * 16) is None          # below '!'
+    assert ol.decode("!" * 15 + "|") is None    # 91 > 90
+    assert ol.decode("!" * 15 + "=") is None    # low part 28 > 26
+
+
+def test_final_score_closed_form(O):
+    # Winner takes the empties; exhaustive over disc counts.
+    for pc in range(0, 65):
+        for oc in range(0, 65 - pc):
+            p = (1 << pc) - 1
+            o = ((1 << oc) - 1) << pc
+            want = 0 if pc == oc else (64 - 2 * oc if pc > oc else 2 * pc - 64)
+            assert O.orc_final_score(p, o) == want
+
+
+def test_next_bit_order(O):
+    # Reference node.cuh:21-28: corners first, then ascending square.
+    assert O.orc_next_bit(0x8100000000000081 | 0x10) == 0x1
+    assert O.orc_next_bit(0x8000000000000000 | 0x10) == 0x8000000000000000
+    assert O.orc_next_bit(0x30) == 0x10
+
+
+@pytest.mark.skipif(not ol.ref_available(), reason="reference host build absent (GPU box)")
+def test_iteration_order_matches_reference(O):
+    import ctypes
+    R = ol.reference()
+    rng = np.random.default_rng(9)
+    for b in ol.generate(5, 3, 50, 20):
+        p, o = int(b[0]), int(b[1])
+        sq = (ctypes.c_int * 64)()
+        n = R.ref_iteration_order(p, o, sq)
+        untried = ~(p | o) & MASK
+        ours = []
+        while untried:
+            bit = O.orc_next_bit(untried)
+            untried ^= bit
+            ours.append(bit.bit_length() - 1)
+        assert ours == list(sq[:n])
+
+
+def test_small_scores_equal_minimax(O):
+    d = np.load(os.path.join(GOLD, "solve_small.npz"))
+    for p, o, s in zip(d["player"], d["opponent"], d["score"]):
+        p, o = int(p), int(o)
+        assert ol.minimax(p, o) == int(s)
+    sc, _, _ = ol.solve_batch(np.stack([d["player"], d["opponent"]], 1), 10)
+    assert np.array_equal(sc, d["score"])
+
+
+def test_mid_fixture_scores(O):
+    d = np.load(os.path.join(GOLD, "solve_mid.npz"))
+    k = 60  # a slice keeps the CPU suite fast
+    sc, _, _ = ol.solve_batch(np.stack([d["player"][:k], d["opponent"][:k]], 1), 14)
+    assert np.array_equal(sc, d["score"][:k])
+
+
+def test_scores_are_even_and_in_range():
+    d = np.load(os.path.join(GOLD, "solve_mid.npz"))
+    s = d["score"].astype(int)
+    assert np.all(np.abs(s) <= 64) and np.all(s % 2 == 0)
+
+
+def test_one_empty_closed_form(O):
+    # One empty: the mover plays it if legal, else the other side, else game over.
+    rng = np.random.default_rng(4)
+    for b in ol.generate(8, 5, 60, 1):
+        p, o = int(b[0]), int(b[1])
+        sq = (~(p | o) & MASK).bit_length() - 1
+        f = O.orc_flip(p, o, sq)
+        if f:
+            want = O.orc_final_score(o ^ f, p ^ f | 1 << sq)
+            want = -want
+        else:
+            g = O.orc_flip(o, p, sq)
+            want = O.orc_final_score(p ^ g, o ^ g | 1 << sq) if g else O.orc_final_score(p, o)
+        assert ol.minimax(p, o) == want
+        assert ol.solve_batch(np.array([[p, o]], dtype=np.uint64), 10)[0][0] == want
+
+
+def test_readme_boards_decode_and_score(O):
+    with open(os.path.join(GOLD, "readme_boards.json")) as f:
+        rows = json.load(f)
+    assert len(rows) == 9
+    for r in rows:
+        p, o = int(r["player"]), int(r["opponent"])
+        assert p & o == 0
+        assert ol.decode(r["text"]) == (p, o)
+        assert ol.empties(p, o) == r["empties"]
+        assert ol.minimax(p, o) == r["score"]
+
+
+def test_generator_is_deterministic_and_exact():
+    a = ol.generate(1, 4, 30, 14)
+    b = ol.generate(1, 4, 30, 14)
+    assert np.array_equal(a, b)
+    for p, o in a:
+        assert int(p) & int(o) == 0
+        assert ol.empties(int(p), int(o)) == 14
+    c = ol.generate(2, 4, 30, 14)
+    assert not np.array_equal(a, c)
+
+
+def test_planted_mistake_is_caught(O):
+    # A search that forgets to negate a child's value must disagree with the fixtures.
+    d = np.load(os.path.join(GOLD, "solve_small.npz"))
+    wrong = 0
+    for p, o, s in zip(d["player"][:80], d["opponent"][:80], d["score"][:80]):
+        p, o = int(p), int(o)
+        m = O.orc_mobility(p, o)
+        if not m:
+            continue
+        best = -65
+        while m:
+            bit = m & -m
+            m ^= bit
+            f = O.orc_flip(p, o, bit.bit_length() - 1)
+            best = max(best, ol.minimax(o ^ f, (p ^ f) | bit))  # sign dropped on purpose
+        wrong += best != int(s)
+    assert wrong > 0
