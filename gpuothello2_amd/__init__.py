@@ -27,6 +27,7 @@ GO2_E_DEVICE = -3
 GO2_E_NOMEM = -4
 GO2_E_STATE = -5
 GO2_E_PARSE = -6
+GO2_E_TIMEOUT = -7
 GO2_MAX_EMPTIES = 32
 
 ORDER_STATIC = 0
@@ -60,7 +61,12 @@ class Config(ctypes.Structure):
         ("learned_min_empties", ctypes.c_int32),
         ("block_size", ctypes.c_int32),
         ("blocks_per_cu", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 11),
+        ("positions_per_wave", ctypes.c_int32),
+        ("split_min_empties", ctypes.c_int32),
+        ("lds_levels", ctypes.c_int32),
+        ("time_limit_ms", ctypes.c_int32),
+        ("split_cut_after", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 6),
     ]
 
 
@@ -71,7 +77,10 @@ class Stats(ctypes.Structure):
         ("score_steps", ctypes.c_uint64),
         ("iterations", ctypes.c_uint64),
         ("kernel_ms", ctypes.c_double),
-        ("reserved", ctypes.c_uint64 * 8),
+        ("steals", ctypes.c_uint64),
+        ("lane_slots", ctypes.c_uint64),
+        ("grid_waves", ctypes.c_uint64),
+        ("reserved", ctypes.c_uint64 * 5),
     ]
 
     def as_dict(self) -> dict:

@@ -43,37 +43,30 @@ int resolve_config(const go2_config* cfg, go2_config* out) {
   if (c.mobility_min_empties < 3) c.mobility_min_empties = 3;
   if (c.learned_min_empties < 3) c.learned_min_empties = 3;
   if (c.blocks_per_cu < 0) return fail(GO2_E_ARG, "config.blocks_per_cu must be >= 0");
+  if (c.positions_per_wave < 0 || c.positions_per_wave > 64)
+    return fail(GO2_E_ARG, "config.positions_per_wave must be in [0, 64]");
+  if (c.split_min_empties < 0) return fail(GO2_E_ARG, "config.split_min_empties must be >= 0");
+  if (c.lds_levels < 0 || c.lds_levels > 30) return fail(GO2_E_ARG, "config.lds_levels must be in [0, 30]");
+  if (c.time_limit_ms < 0) return fail(GO2_E_ARG, "config.time_limit_ms must be >= 0");
+  if (c.split_cut_after < 0 || c.split_cut_after > 3)
+    return fail(GO2_E_ARG, "config.split_cut_after must be in [0, 3]");
   *out = c;
   return GO2_OK;
 }
 
-// Device buffers of one solve: counters plus, for host-array calls, the
-// boards and scores.
-struct DeviceBufs {
-  SolveCounters* ctr = nullptr;
+// Host-array calls also need device copies of the boards and scores.
+struct IoBufs {
   go2_board* boards = nullptr;
   int8_t* scores = nullptr;
-  size_t cap = 0;
-  ~DeviceBufs() {
-    if (ctr) (void)hipFree(ctr);
+  ~IoBufs() {
     if (boards) (void)hipFree(boards);
     if (scores) (void)hipFree(scores);
   }
-  int alloc(size_t n, bool io) {
+  int alloc(size_t n) {
     hipError_t e;
-    if (!ctr && (e = hipMalloc(&ctr, sizeof(SolveCounters))) != hipSuccess)
-      return hip_fail(e, "hipMalloc(counters)");
-    if (io && n > cap) {
-      if (boards) (void)hipFree(boards);
-      if (scores) (void)hipFree(scores);
-      boards = nullptr;
-      scores = nullptr;
-      cap = 0;
-      if ((e = hipMalloc(&boards, n * sizeof(go2_board))) != hipSuccess)
-        return hip_fail(e, "hipMalloc(boards)");
-      if ((e = hipMalloc(&scores, n)) != hipSuccess) return hip_fail(e, "hipMalloc(scores)");
-      cap = n;
-    }
+    if ((e = hipMalloc(&boards, n * sizeof(go2_board))) != hipSuccess)
+      return hip_fail(e, "hipMalloc(boards)");
+    if ((e = hipMalloc(&scores, n)) != hipSuccess) return hip_fail(e, "hipMalloc(scores)");
     return GO2_OK;
   }
 };
@@ -108,7 +101,8 @@ struct go2_task {
   int max_depth = 0;
   go2_config cfg{};
   hipStream_t stream = nullptr;
-  DeviceBufs bufs;
+  Workspace ws;
+  IoBufs bufs;
   Events ev;
   bool launched = false;
 };
@@ -133,6 +127,10 @@ void go2_config_default(go2_config* cfg) {
   cfg->learned_min_empties = 12;
   cfg->block_size = 256;
   cfg->blocks_per_cu = 0;
+  cfg->positions_per_wave = 0;
+  cfg->split_min_empties = 0;
+  cfg->lds_levels = 0;
+  cfg->time_limit_ms = 600000;
 }
 
 // base81: character i encodes squares 4i..4i+3 as 33 + d0 + 3 d1 + 9 d2 + 32 d3,
@@ -178,17 +176,15 @@ int go2_solve_device(const go2_board* d_boards, size_t n, int max_depth, int8_t*
   go2_config c;
   if ((rc = resolve_config(cfg, &c)) != GO2_OK) return rc;
   hipStream_t s = (hipStream_t)hip_stream;
-  DeviceBufs bufs;
-  Events ev;
-  if ((rc = bufs.alloc(0, false)) != GO2_OK) return rc;
-  if ((rc = ev.create()) != GO2_OK) return rc;
   if (n == 0) {
     if (stats) *stats = go2_stats{};
     return GO2_OK;
   }
-  if ((rc = launch_solve(d_boards, n, max_depth, d_scores, c, bufs.ctr, s, ev.a, ev.b)) != GO2_OK)
-    return rc;
-  return collect_stats(bufs.ctr, n, s, ev.a, ev.b, stats);
+  Workspace ws;
+  Events ev;
+  if ((rc = ev.create()) != GO2_OK) return rc;
+  if ((rc = launch_solve(d_boards, n, max_depth, d_scores, c, ws, s, ev.a, ev.b)) != GO2_OK) return rc;
+  return collect_stats(ws, s, ev.a, ev.b, stats);
 }
 
 int go2_solve(const go2_board* boards, size_t n, int max_depth, int8_t* scores,
@@ -218,7 +214,7 @@ go2_task* go2_task_create(size_t n, int max_depth, const go2_config* cfg) {
     delete t;
     return nullptr;
   }
-  if (t->bufs.alloc(n ? n : 1, true) != GO2_OK || t->ev.create() != GO2_OK) {
+  if (t->bufs.alloc(n ? n : 1) != GO2_OK || t->ev.create() != GO2_OK) {
     (void)hipStreamDestroy(t->stream);
     delete t;
     return nullptr;
@@ -245,7 +241,7 @@ int go2_task_launch(go2_task* t) {
   if (!t) return fail(GO2_E_ARG, "null task");
   int rc = GO2_OK;
   if (t->count)
-    rc = launch_solve(t->bufs.boards, t->count, t->max_depth, t->bufs.scores, t->cfg, t->bufs.ctr,
+    rc = launch_solve(t->bufs.boards, t->count, t->max_depth, t->bufs.scores, t->cfg, t->ws,
                       t->stream, t->ev.a, t->ev.b);
   if (rc == GO2_OK) t->launched = true;
   return rc;
@@ -276,7 +272,7 @@ int go2_task_download(go2_task* t, int8_t* scores, go2_stats* stats) {
     if (stats) *stats = go2_stats{};
     return GO2_OK;
   }
-  int rc = collect_stats(t->bufs.ctr, t->count, t->stream, t->ev.a, t->ev.b, stats);
+  int rc = collect_stats(t->ws, t->stream, t->ev.a, t->ev.b, stats);
   if (rc != GO2_OK) return rc;
   hipError_t e = hipMemcpyAsync(scores, t->bufs.scores, t->count, hipMemcpyDeviceToHost, t->stream);
   if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(scores)");

@@ -13,12 +13,16 @@ namespace go2 {
 // memset node, so the launch sequence stays capturable in a HIP graph).
 struct SolveCounters {
   unsigned int next;   // work-queue head: next board index to hand out
-  unsigned int error;  // 1 + index of the first invalid board, 0 if none
+  unsigned int error;  // 1 + index of the first invalid board, kTimedOut, or 0
   unsigned long long positions;
   unsigned long long nodes;
   unsigned long long score_steps;
   unsigned long long iterations;
+  unsigned long long steals;
+  unsigned long long lane_slots;
 };
+
+constexpr unsigned int kTimedOut = 0xFFFFFFFFu;
 
 // Records a failure for go2_last_error() and returns `code`.
 int fail(int code, const std::string& msg);
@@ -27,15 +31,29 @@ int hip_fail(hipError_t e, const char* what);
 // Validates cfg (or fills defaults when null) into `out`.
 int resolve_config(const go2_config* cfg, go2_config* out);
 
+// Device memory of a solve that outlives one launch: counters and the
+// per-lane search stacks kept in HBM.  Grown on demand, freed on destruction.
+struct Workspace {
+  SolveCounters* ctr = nullptr;
+  void* frames = nullptr;
+  size_t frames_bytes = 0;
+  unsigned long long grid_waves = 0;  // of the last launch
+  Workspace() = default;
+  Workspace(const Workspace&) = delete;
+  Workspace& operator=(const Workspace&) = delete;
+  ~Workspace();
+  int ensure(size_t frame_bytes);
+};
+
 // Enqueue counter reset + solve kernel on `stream`; events bracket the solve
 // kernel when non-null.
 int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_scores,
-                 const go2_config& cfg, SolveCounters* d_ctr, hipStream_t stream, hipEvent_t ev0,
+                 const go2_config& cfg, Workspace& ws, hipStream_t stream, hipEvent_t ev0,
                  hipEvent_t ev1);
 
 // Copy counters back (stream-ordered, then synchronises the stream) and fill stats.
-int collect_stats(SolveCounters* d_ctr, size_t n, hipStream_t stream, hipEvent_t ev0,
-                  hipEvent_t ev1, go2_stats* stats);
+int collect_stats(Workspace& ws, hipStream_t stream, hipEvent_t ev0, hipEvent_t ev1,
+                  go2_stats* stats);
 
 int launch_generate(uint64_t seed, uint32_t config_id, size_t n, int emin, int emax,
                     go2_board* d_out, hipStream_t stream);

@@ -3,30 +3,45 @@
 // What it replaces: alpha_beta_kernel and the Solver state machine of the
 // reference (solver.cu:15-311), which runs one CUDA thread per board with
 // 80-byte global-memory frames for the top DEPTH-9 plies and 24-byte
-// shared-memory frames for the last 10.
+// shared-memory frames for the last 10, and a per-block stride queue.
 //
-// Design here (wave64, one lane per board):
-//   * Persistent grid sized to residency; each lane pulls the next board
-//     index from one device counter when its board is done, so lanes never
-//     idle while boards remain (replaces the per-block stride queue,
-//     solver.cu:169-177,289-292).
-//   * The node being searched lives in VGPRs.  Only ancestors are spilled to
-//     LDS, one frame per empties level, struct-of-arrays with the lane index
-//     fastest so every frame access is bank-conflict free whatever depth each
-//     lane is at.  A frame stores the flip set of the move taken (the parent
-//     board is rebuilt from the child by two XORs), the untried-move set, the
-//     packed move order, and alpha/beta/best/move/pass in one dword:
-//     28 bytes per level, levels for 3..MAXE empties.
-//   * Every loop iteration does the same heavy work on every active lane:
-//     one flip and one mobility (or, one ply from the end, the exact score of
-//     the last empty square).  Move ordering work is spread over iterations
-//     ("scoring" steps: one child per iteration), so lanes of a wave that are
-//     ordering and lanes that are descending run the same instructions.
-//   * Exact fail-soft negamax, full window (-64, 64); passes are folded into
-//     the child frame (board swapped, window kept, value not negated), which
-//     is the same recurrence as solver.cu:65-73,162-165,214-221,246-256.
-// Scores do not depend on move order, so any ordering gives the reference's
-// scores bit for bit; ordering only changes node counts.
+// Design here (wave64):
+//   * Persistent grid sized to what stays resident on every CU.  Each
+//     wavefront owns up to K boards at a time (K = positions_per_wave, from
+//     the batch size so that every wavefront gets work); owner lanes pull
+//     board indices from one device counter with one atomic per wavefront.
+//   * Lanes with no board of their own help: once a node with at least
+//     split_min empties has searched its first child (young brothers wait),
+//     an idle lane of the same wavefront takes the next untried sibling,
+//     searches it with the node's current window and merges its fail-soft
+//     value into the node (max), while the owner keeps searching the other
+//     siblings.  The owner returns from the node only when every taken
+//     sibling has reported.  A merged value with a window that was narrower
+//     when the sibling started is still a correct fail-soft bound, so the
+//     exact root score is unchanged (any order, any split gives the same
+//     minimax value).  All of this stays inside one wavefront: hand-offs,
+//     results and stop requests travel by ballot and readlane in
+//     wavefront-uniform phases, and every frame is touched only by the lane
+//     that owns it, so no atomics or fences are needed.  A node that is cut
+//     while helpers still search its siblings stops them (and their own
+//     helpers, one level per loop trip) instead of waiting them out.
+//   * The node being searched lives in VGPRs.  Ancestors are spilled to a
+//     stack, one frame per empties level: levels 3..(lds_levels+2) in LDS
+//     (struct-of-arrays, lane index fastest, bank-conflict free), higher
+//     levels in HBM (same layout over all lanes of the grid; touched only
+//     by the rare nodes near the root).  A frame holds the flip set of the
+//     move taken (the parent board is rebuilt from the child by XORs), the
+//     untried moves, the packed move order and alpha/beta/best/move/flags.
+//     Nodes at split levels also keep their board, a pending-helper count
+//     and the best value helpers reported, in HBM.
+//   * Every loop iteration does the same heavy work on every searching
+//     lane: one flip and one mobility (or, one ply from the end, the exact
+//     score of the last empty square).  Move ordering work is spread over
+//     iterations ("scoring" steps: one child per iteration).
+//   * Exact fail-soft negamax, full window (-64, 64) at the root; passes
+//     are folded into the child frame (board swapped, window kept, value not
+//     negated), the recurrence of solver.cu:65-73,162-165,214-221,246-256.
+//   * A wall-clock deadline (s_memrealtime) bounds every launch.
 #include "bitboard.hpp"
 #include "internal.hpp"
 
@@ -57,204 +72,558 @@ __device__ __forceinline__ int pick_parity(uint64_t m, uint64_t empty) {
   return ctz64(c ? c : (po ? po : (nx ? nx : m)));
 }
 
-// Packed sorted move list: up to 9 entries of 7 bits (63 bits).  Keys and
-// squares are kept in two such words; comparisons are done field-parallel.
+template <int ORDER>
+__device__ __forceinline__ int pick_static(uint64_t m, uint64_t empty) {
+  if constexpr (ORDER == 0) {
+    return pick_plain(m);
+  } else {
+    return pick_parity(m, empty);
+  }
+}
+
+// Packed sorted move list: up to 9 entries of 7 bits (63 bits), entry =
+// square + 1, first entry in the low bits.  Keys are kept the same way and
+// compared field-parallel.
 constexpr uint64_t kLo7 = 0x0102040810204081ULL;  // bit 0 of each 7-bit field
-constexpr uint64_t kHi7 = kLo7 << 6;                                         // bit 6 of each field
+constexpr uint64_t kHi7 = kLo7 << 6;              // bit 6 of each field
 constexpr uint64_t kFull63 = 0x7FFFFFFFFFFFFFFFULL;
 static_assert(__builtin_popcountll(kLo7) == 9, "nine 7-bit fields");
 
-__device__ __forceinline__ uint32_t pack_frame(int alpha, int beta, int best, int sq, bool passed) {
-  return (uint32_t)(uint8_t)alpha | ((uint32_t)(uint8_t)beta << 8) |
-         ((uint32_t)(uint8_t)best << 16) | ((uint32_t)sq << 24) | ((uint32_t)passed << 30);
+// Frame word.  Every window bound and value in this search is even (final
+// scores are even, the root window is (-64, 64) and only max / negation
+// combine them), so alpha, beta and best are kept halved in 7 bits each
+// (bits 0-6, 7-13, 14-20); then the move square (21-26), the
+// child-was-a-pass flag (27), the number of finished children saturated at
+// 3 (28-29) and the expected node type (30-31).
+enum NodeType { kPV = 0, kCut = 1, kAll = 2 };
+
+__device__ __forceinline__ uint32_t pack_frame(int alpha, int beta, int best, int sq, bool passed,
+                                               int done, int type) {
+  return ((uint32_t)(alpha >> 1) & 127u) | (((uint32_t)(beta >> 1) & 127u) << 7) |
+         (((uint32_t)(best >> 1) & 127u) << 14) | ((uint32_t)sq << 21) | ((uint32_t)passed << 27) |
+         ((uint32_t)done << 28) | ((uint32_t)type << 30);
 }
+
+// Even value held halved in the 7-bit field starting at `shift`.
+__device__ __forceinline__ int frame_val(uint32_t w, int shift) {
+  return ((int)(w << (25 - shift)) >> 25) * 2;
+}
+
+// Expected type of a child: the first child of a PV node is PV, its other
+// children are CUT nodes; children of CUT nodes are ALL nodes and children
+// of ALL nodes are CUT nodes.
+__device__ __forceinline__ int child_type(int type, int done) {
+  return type == kPV ? (done == 0 ? kPV : kCut) : (type == kCut ? kAll : kCut);
+}
+
+constexpr int kNoValue = -127;  // "no helper reported yet"
 
 struct SolveArgs {
   const go2_board* boards;
   int8_t* scores;
   unsigned int n;
-  int depth;   // DEPTH: boards with more empties are rejected
-  int kff;     // fastest-first ordering at nodes with >= kff empties
+  int depth;          // DEPTH: boards with more empties are rejected
+  int kff;            // fastest-first ordering at nodes with >= kff empties
+  int split_min;      // helpers take siblings at nodes with >= split_min empties
+  int cut_after;      // ... at expected CUT nodes only after this many children
+  int lds_levels;     // frame levels 3..lds_levels+2 live in LDS
+  int K;              // boards owned by one wavefront at a time
+  unsigned int total_lanes;
+  unsigned long long limit_ticks;  // time limit in s_memrealtime ticks (100 MHz); 0 = none
   SolveCounters* ctr;
+  // HBM frames of levels >= lds_levels+3: [level - lds_levels - 3][total_lanes]
+  uint64_t* g_fb;
+  uint64_t* g_mv;
+  uint64_t* g_od;
+  uint32_t* g_wd;
+  uint32_t* g_sx;
 };
+
+// Split word of a frame (levels >= split_min): helpers still out on this
+// node (bits 0-7) and the best value they reported, halved and offset by 64
+// (bits 8-15; 0 = none reported).  Only the owning lane reads or writes it.
+__device__ __forceinline__ int sx_pending(uint32_t x) { return (int)(x & 255u); }
+__device__ __forceinline__ int sx_best(uint32_t x) {
+  const int b = (int)((x >> 8) & 255u);
+  return b ? (b - 64) * 2 : kNoValue;
+}
+__device__ __forceinline__ uint32_t sx_with_best(uint32_t x, int v) {
+  const uint32_t b = (uint32_t)(v / 2 + 64);
+  return ((x >> 8) & 255u) >= b ? x : ((x & ~0xFF00u) | (b << 8));
+}
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t x, int lane) {
+  const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)x, lane);
+  const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), lane);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// HBM frame accesses.  Written as relaxed wavefront-scope atomics (plain
+// global loads/stores in the ISA) so that the compiler cannot fold an
+// LDS-or-HBM pair into one slower flat access.
+template <class T>
+__device__ __forceinline__ T hbm_ld(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+template <class T>
+__device__ __forceinline__ void hbm_st(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
+// Board of the parent of (P, O), from the frame word and flip set the parent
+// stored when it moved to (P, O).
+__device__ __forceinline__ void undo_move(uint64_t& P, uint64_t& O, uint64_t fb, uint32_t w) {
+  const uint64_t bit = 1ULL << ((w >> 21) & 63);
+  if ((w >> 27) & 1) {  // the child was a pass frame: same side to move
+    const uint64_t nP = P ^ fb;
+    O = O ^ fb ^ bit;
+    P = nP;
+  } else {
+    const uint64_t nP = O ^ fb;
+    O = P ^ fb ^ bit;
+    P = nP;
+  }
+}
 
 template <int MAXE, int ORDER>
 __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
   extern __shared__ uint64_t lds[];
-  constexpr int L = MAXE - 2;  // frame levels: nodes with 3..MAXE empties
   const int bs = blockDim.x, tid = threadIdx.x;
-  uint64_t* fbA = lds;
-  uint64_t* mvA = fbA + L * bs;
-  uint64_t* odA = mvA + L * bs;
-  uint32_t* wdA = reinterpret_cast<uint32_t*>(odA + L * bs);
+  const int lane = __lane_id();
+  const unsigned int gtid = blockIdx.x * bs + tid;
+  const int LL = a.lds_levels;
+  uint64_t* l_fb = lds;
+  uint64_t* l_mv = l_fb + LL * bs;
+  uint64_t* l_od = l_mv + LL * bs;
+  uint32_t* l_wd = reinterpret_cast<uint32_t*>(l_od + LL * bs);
+  uint32_t* l_sx = l_wd + LL * bs;
 
+  // Every frame slot is read and written only by the lane that owns it.
+  // LDS levels use ds_* accesses and HBM levels global ones (no flat
+  // addressing on the hot path).
+  auto lds_k = [&](int E) { return (E - 3) * bs + tid; };
+  auto hbm_k = [&](int E) { return (size_t)(E - 3 - LL) * a.total_lanes + gtid; };
+  auto in_lds = [&](int E) { return E - 3 < LL; };
+  auto get_sx = [&](int E) -> uint32_t { return in_lds(E) ? l_sx[lds_k(E)] : hbm_ld(&a.g_sx[hbm_k(E)]); };
+  auto set_sx = [&](int E, uint32_t v) {
+    if (in_lds(E)) {
+      l_sx[lds_k(E)] = v;
+    } else {
+      hbm_st(&a.g_sx[hbm_k(E)], v);
+    }
+  };
+  auto get_fb = [&](int E) -> uint64_t { return in_lds(E) ? l_fb[lds_k(E)] : hbm_ld(&a.g_fb[hbm_k(E)]); };
+  auto get_mv = [&](int E) -> uint64_t { return in_lds(E) ? l_mv[lds_k(E)] : hbm_ld(&a.g_mv[hbm_k(E)]); };
+  auto get_od = [&](int E) -> uint64_t { return in_lds(E) ? l_od[lds_k(E)] : hbm_ld(&a.g_od[hbm_k(E)]); };
+  auto get_wd = [&](int E) -> uint32_t { return in_lds(E) ? l_wd[lds_k(E)] : hbm_ld(&a.g_wd[hbm_k(E)]); };
+  auto set_mv_od = [&](int E, uint64_t mv, uint64_t od) {
+    if (in_lds(E)) {
+      l_mv[lds_k(E)] = mv;
+      l_od[lds_k(E)] = od;
+    } else {
+      hbm_st(&a.g_mv[hbm_k(E)], mv);
+      hbm_st(&a.g_od[hbm_k(E)], od);
+    }
+  };
+  auto push_frame = [&](int E, uint64_t fb, uint64_t mv, uint64_t od, uint32_t wd) {
+    if (in_lds(E)) {
+      const int k = lds_k(E);
+      l_fb[k] = fb;
+      l_mv[k] = mv;
+      l_od[k] = od;
+      l_wd[k] = wd;
+    } else {
+      const size_t k = hbm_k(E);
+      hbm_st(&a.g_fb[k], fb);
+      hbm_st(&a.g_mv[k], mv);
+      hbm_st(&a.g_od[k], od);
+      hbm_st(&a.g_wd[k], wd);
+    }
+  };
+
+  // Lane state.
+  enum { IDLE = 0, SEARCH = 1, WAIT = 2, ABORT = 3, REPORT = 4 };
+  int mode = IDLE;
   uint64_t P = 0, O = 0, moves = 0, order = 0, to_score = 0, keys = 0;
   int alpha = 0, beta = 0, best = 0, E = 0, Eroot = 0, nkeys = 0;
-  bool scoring = false, negate = false, active = false;
+  bool scoring = false, negate = false;
+  int ndone = 0, ntype = kPV;   // finished children (saturating at 3) and expected type
+  int owner = -1, owner_E = 0;  // owner lane and frame level of a helper's task (-1: own board)
   unsigned int idx = 0;
-  unsigned long long c_pos = 0, c_nodes = 0, c_steps = 0, c_iters = 0;
+  uint32_t split_mask = 0;      // bit E-3: this lane's frame at level E offers siblings
+  int helpers_out = 0;          // helpers still working on this lane's frames
+  int dead_level = 64;          // frame whose helpers must stop: E, 0 = all, 64 = none
+  bool dead_new = false;        // dead_level changed; helpers not told yet
+  int report_value = 0;
+  bool report_aborted = false;
+  bool queue_done = false;      // wavefront-uniform
+  unsigned long long c_pos = 0, c_nodes = 0, c_steps = 0, c_iters = 0, c_steals = 0, c_slots = 0;
 
+  // A node at level E with legal moves `cm` becomes the current node.
+  auto enter_node = [&](uint64_t cm, int type) {
+    moves = cm;
+    best = -64;
+    order = 0;
+    scoring = ORDER >= 1 && E >= a.kff && (cm & (cm - 1));
+    to_score = cm;
+    keys = 0;
+    nkeys = 0;
+    ndone = 0;
+    ntype = type;
+    if (E >= a.split_min) set_sx(E, 0u);
+  };
+
+  unsigned int trip = 0;
+  const unsigned long long deadline = __builtin_amdgcn_s_memrealtime() + a.limit_ticks;
   for (;;) {
-    if (!active) {
-      idx = atomicAdd(&a.ctr->next, 1u);
-      if (idx >= a.n) break;
-      const go2_board b = a.boards[idx];
-      P = b.player;
-      O = b.opponent;
-      E = 64 - popc64(P | O);
-      ++c_pos;
-      ++c_nodes;
-      if ((P & O) || E > a.depth) {
-        atomicCAS(&a.ctr->error, 0u, idx + 1);
-        a.scores[idx] = 0;
-        continue;
-      }
-      if (E <= 1) {
-        a.scores[idx] = (int8_t)(E == 0 ? final_score(P, O) : last_score(P, O, ctz64(~(P | O))));
-        continue;
-      }
-      moves = mobility(P, O);
-      negate = false;
-      if (!moves) {
-        const uint64_t om = mobility(O, P);
-        if (!om) {
-          a.scores[idx] = (int8_t)final_score(P, O);
-          continue;
-        }
-        const uint64_t t = P;
-        P = O;
-        O = t;
-        moves = om;
-        negate = true;
-      }
-      Eroot = E;
-      alpha = -64;
-      beta = 64;
-      best = -64;
-      order = 0;
-      scoring = ORDER >= 1 && E >= a.kff && (moves & (moves - 1));
-      to_score = moves;
-      keys = 0;
-      nkeys = 0;
-      active = true;
-      continue;
-    }
-
-    ++c_iters;
-    int sq;
-    if (scoring) {
-      sq = ctz64(to_score);
-    } else if (order) {
-      sq = (int)(order & 127) - 1;
-      order >>= 7;
-    } else {
-      sq = ORDER == 0 ? pick_plain(moves) : pick_parity(moves, ~(P | O));
-    }
-    const uint64_t bit = 1ULL << sq;
-    const uint64_t f = flip(P, O, sq);
-    const uint64_t cP = O ^ f, cO = P ^ f ^ bit;
-    const bool leaf = !scoring && E == 2;
-    uint64_t cm = 0;
-    int v = 0;
-    if (leaf) {
-      v = -last_score(cP, cO, ctz64(~(cP | cO)));
-    } else {
-      cm = mobility(cP, cO);
-    }
-
-    if (scoring) {
-      // Fastest first: fewest replies, corners preferred, X-squares delayed.
-      ++c_steps;
-      const int k = min(popc64(cm), 15) * 4 + ((bit & kCorners) ? 0 : 2) + ((bit & kXSquares) ? 1 : 0);
-      const uint64_t valid = (1ULL << (7 * nkeys)) - 1;
-      const uint64_t le = ((((uint64_t)k * kLo7) | kHi7) - keys) & kHi7 & valid;
-      const int c = popc64(le);  // entries with key <= k: insert after them (stable)
-      const uint64_t low = (1ULL << (7 * c)) - 1;
-      keys = (((keys & ~low) << 7) | (keys & low) | ((uint64_t)k << (7 * c))) & kFull63;
-      order = (((order & ~low) << 7) | (order & low) | ((uint64_t)(sq + 1) << (7 * c))) & kFull63;
-      nkeys = min(nkeys + 1, 9);
-      to_score ^= bit;
-      scoring = to_score != 0;
-      continue;
-    }
-
-    moves ^= bit;
-    ++c_nodes;
-    if (!leaf) {
-      bool passed = false;
-      bool terminal = false;
-      if (!cm) {
-        const uint64_t om = mobility(cO, cP);
-        if (!om) {
-          v = -final_score(cP, cO);
-          terminal = true;
-        } else {
-          cm = om;
-          passed = true;
-        }
-      }
-      if (!terminal) {
-        const int at = (E - 3) * bs + tid;
-        fbA[at] = f | bit;
-        mvA[at] = moves;
-        odA[at] = order;
-        wdA[at] = pack_frame(alpha, beta, best, sq, passed);
-        if (passed) {
-          P = cO;
-          O = cP;
-        } else {
-          P = cP;
-          O = cO;
-          const int t = alpha;
-          alpha = -beta;
-          beta = -t;
-        }
-        moves = cm;
-        best = -64;
-        --E;
-        order = 0;
-        scoring = ORDER >= 1 && E >= a.kff && (cm & (cm - 1));
-        to_score = cm;
-        keys = 0;
-        nkeys = 0;
-        continue;
-      }
-    }
-
-    // A child value is known: commit it, then return from every finished node.
-    best = max(best, v);
-    alpha = max(alpha, best);
-    while (alpha >= beta || !moves) {
-      if (E == Eroot) {
-        a.scores[idx] = (int8_t)(negate ? -best : best);
-        active = false;
+    // ---- time limit (checked every 256 trips; uniform per wavefront) ----
+    if (a.limit_ticks && (++trip & 255) == 0) {
+      if (__builtin_amdgcn_s_memrealtime() > deadline) {
+        if (lane == 0) atomicExch(&a.ctr->error, kTimedOut);
         break;
       }
-      ++E;
-      const int at = (E - 3) * bs + tid;
-      const uint64_t fb = fbA[at];
-      moves = mvA[at];
-      order = odA[at];
-      const uint32_t w = wdA[at];
-      const int palpha = (int)(int8_t)(w & 0xFF);
-      const int pbeta = (int)(int8_t)((w >> 8) & 0xFF);
-      const int pbest = (int)(int8_t)((w >> 16) & 0xFF);
-      const uint64_t pbit = 1ULL << ((w >> 24) & 63);
-      int vv;
-      if ((w >> 30) & 1) {  // child was a pass frame: same side to move as here
-        const uint64_t nP = P ^ fb;
-        O = O ^ fb ^ pbit;
-        P = nP;
-        vv = best;
-      } else {
-        const uint64_t nP = O ^ fb;
-        O = P ^ fb ^ pbit;
-        P = nP;
-        vv = -best;
+    }
+
+    // ---- 1. owner lanes fetch boards: one atomic per wavefront ----
+    if (!queue_done) {
+      const uint64_t want = __ballot(mode == IDLE && lane < a.K);
+      if (want) {
+        const int leader = __builtin_ctzll(want);
+        const unsigned int cnt = (unsigned int)__builtin_popcountll(want);
+        unsigned int base = 0;
+        if (lane == leader) base = atomicAdd(&a.ctr->next, cnt);
+        base = (unsigned int)__builtin_amdgcn_readlane((int)base, leader);
+        if (base + cnt >= a.n) queue_done = true;
+        if ((want >> lane) & 1) {
+          const unsigned int i = base + (unsigned int)__builtin_popcountll(want & __lanemask_lt());
+          if (i < a.n) {
+            idx = i;
+            const go2_board b = a.boards[i];
+            P = b.player;
+            O = b.opponent;
+            E = 64 - popc64(P | O);
+            ++c_pos;
+            ++c_nodes;
+            if ((P & O) || E > a.depth) {
+              atomicCAS(&a.ctr->error, 0u, i + 1);
+              a.scores[i] = 0;
+            } else if (E <= 1) {
+              a.scores[i] = (int8_t)(E == 0 ? final_score(P, O) : last_score(P, O, ctz64(~(P | O))));
+            } else {
+              uint64_t m = mobility(P, O);
+              negate = false;
+              bool done = false;
+              if (!m) {
+                const uint64_t om = mobility(O, P);
+                if (!om) {
+                  a.scores[i] = (int8_t)final_score(P, O);
+                  done = true;
+                } else {
+                  const uint64_t t = P;
+                  P = O;
+                  O = t;
+                  m = om;
+                  negate = true;
+                }
+              }
+              if (!done) {
+                Eroot = E;
+                alpha = -64;
+                beta = 64;
+                owner = -1;
+                enter_node(m, kPV);
+                mode = SEARCH;
+              }
+            }
+          }
+        }
       }
-      beta = pbeta;
-      best = max(pbest, vv);
-      alpha = max(palpha, best);
+    } else if (__ballot(mode != IDLE) == 0) {
+      break;
+    }
+
+    // ---- 2. helpers report to their owners, one at a time ----
+    uint64_t rm = __ballot(mode == REPORT);
+    while (rm) {
+      const int r = __builtin_ctzll(rm);
+      rm &= rm - 1;
+      const int o = __builtin_amdgcn_readlane(owner, r);
+      const int oe = __builtin_amdgcn_readlane(owner_E, r);
+      const int val = __builtin_amdgcn_readlane(report_value, r);
+      const int ab = __builtin_amdgcn_readlane((int)report_aborted, r);
+      if (lane == o) {
+        uint32_t sx = get_sx(oe) - 1u;
+        if (!ab) sx = sx_with_best(sx, val);
+        set_sx(oe, sx);
+        --helpers_out;
+      }
+      if (lane == r) mode = IDLE;
+    }
+
+    // ---- 3. cut frames stop their helpers (cascades one level per trip) ----
+    uint64_t dm = __ballot(dead_new);
+    while (dm) {
+      const int l = __builtin_ctzll(dm);
+      dm &= dm - 1;
+      const int dl = __builtin_amdgcn_readlane(dead_level, l);
+      if (lane == l) dead_new = false;
+      if ((mode == SEARCH || mode == WAIT) && owner == l && (dl == 0 || dl == owner_E)) {
+        mode = ABORT;
+        dead_level = 0;
+        dead_new = true;
+        split_mask = 0;
+      }
+    }
+
+    // ---- 4. an idle lane takes a sibling from the lane with the highest offer ----
+    const uint64_t idle = __ballot(mode == IDLE);
+    if (idle && __ballot(split_mask != 0)) {
+      uint32_t lv = split_mask;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) lv |= (uint32_t)__shfl_xor((int)lv, o);
+      lv = (uint32_t)__builtin_amdgcn_readfirstlane((int)lv);
+      const int e = 31 - __builtin_clz(lv) + 3;
+      const int victim = __builtin_ctzll(__ballot((split_mask >> (e - 3)) & 1));
+      const int thief = __builtin_ctzll(idle);
+      int give = 0, ca = 0, cb = 0, ct = 0;
+      uint64_t cP = 0, cO = 0;
+      if (lane == victim) {
+        uint64_t mv = get_mv(e), od = get_od(e);
+        const uint32_t w = get_wd(e);
+        const uint32_t sx = get_sx(e);
+        const int fa = frame_val(w, 0), fbeta = frame_val(w, 7);
+        const int am = max(fa, sx_best(sx));
+        if (mv == 0 || am >= fbeta) {
+          split_mask &= ~(1u << (e - 3));
+        } else {
+          // Rebuild the board of the node at level e from the current node.
+          uint64_t bp = P, bo = O;
+          for (int l = E + 1; l <= e; ++l) undo_move(bp, bo, get_fb(l), get_wd(l));
+          int sq;
+          if (od) {
+            sq = (int)(od & 127) - 1;
+            od >>= 7;
+          } else {
+            sq = pick_static<ORDER>(mv, ~(bp | bo));
+          }
+          const uint64_t bit = 1ULL << sq;
+          mv ^= bit;
+          set_mv_od(e, mv, od);
+          set_sx(e, sx + 1u);
+          ++helpers_out;
+          if (!mv) split_mask &= ~(1u << (e - 3));
+          const uint64_t f = flip(bp, bo, sq);
+          cP = bo ^ f;
+          cO = bp ^ f ^ bit;
+          ca = am;
+          cb = fbeta;
+          ct = child_type((int)(w >> 30), 1);
+          give = 1;
+        }
+      }
+      give = __builtin_amdgcn_readlane(give, victim);
+      if (give) {
+        cP = readlane64(cP, victim);
+        cO = readlane64(cO, victim);
+        ca = __builtin_amdgcn_readlane(ca, victim);
+        cb = __builtin_amdgcn_readlane(cb, victim);
+        ct = __builtin_amdgcn_readlane(ct, victim);
+        if (lane == thief) {
+          ++c_steals;
+          ++c_nodes;
+          E = e - 1;
+          Eroot = E;
+          owner = victim;
+          owner_E = e;
+          dead_level = 64;
+          const uint64_t cm = mobility(cP, cO);
+          if (cm) {
+            P = cP;
+            O = cO;
+            alpha = -cb;
+            beta = -ca;
+            negate = true;
+            enter_node(cm, ct);
+            mode = SEARCH;
+          } else {
+            const uint64_t om = mobility(cO, cP);
+            if (om) {
+              P = cO;
+              O = cP;
+              alpha = ca;
+              beta = cb;
+              negate = false;
+              enter_node(om, ct);
+              mode = SEARCH;
+            } else {
+              report_value = -final_score(cP, cO);
+              report_aborted = false;
+              mode = REPORT;
+            }
+          }
+        }
+      }
+    }
+
+    // ---- 5. one search step ----
+    c_slots += 1;
+    bool commit = false;
+    int v = 0;
+    if (mode == SEARCH) {
+      ++c_iters;
+      int sq;
+      if (scoring) {
+        sq = ctz64(to_score);
+      } else if (order) {
+        sq = (int)(order & 127) - 1;
+        order >>= 7;
+      } else {
+        sq = pick_static<ORDER>(moves, ~(P | O));
+      }
+      const uint64_t bit = 1ULL << sq;
+      const uint64_t f = flip(P, O, sq);
+      const uint64_t cP = O ^ f, cO = P ^ f ^ bit;
+      const bool leaf = !scoring && E == 2;
+      uint64_t cm = 0;
+      if (leaf) {
+        v = -last_score(cP, cO, ctz64(~(cP | cO)));
+      } else {
+        cm = mobility(cP, cO);
+      }
+
+      if (scoring) {
+        // Fastest first: fewest replies, corners preferred, X-squares delayed.
+        ++c_steps;
+        const int k = min(popc64(cm), 15) * 4 + ((bit & kCorners) ? 0 : 2) + ((bit & kXSquares) ? 1 : 0);
+        const uint64_t valid = (1ULL << (7 * nkeys)) - 1;
+        const uint64_t le = ((((uint64_t)k * kLo7) | kHi7) - keys) & kHi7 & valid;
+        const int c = popc64(le);  // entries with key <= k: insert after them (stable)
+        const uint64_t low = (1ULL << (7 * c)) - 1;
+        keys = (((keys & ~low) << 7) | (keys & low) | ((uint64_t)k << (7 * c))) & kFull63;
+        order = (((order & ~low) << 7) | (order & low) | ((uint64_t)(sq + 1) << (7 * c))) & kFull63;
+        nkeys = min(nkeys + 1, 9);
+        to_score ^= bit;
+        scoring = to_score != 0;
+      } else {
+        moves ^= bit;
+        ++c_nodes;
+        commit = leaf;
+        if (!leaf) {
+          bool passed = false;
+          if (!cm) {
+            const uint64_t om = mobility(cO, cP);
+            if (!om) {
+              v = -final_score(cP, cO);
+              commit = true;
+            } else {
+              cm = om;
+              passed = true;
+            }
+          }
+          if (!commit) {
+            push_frame(E, f | bit, moves, order, pack_frame(alpha, beta, best, sq, passed, ndone, ntype));
+            if (E >= a.split_min && moves && ndone >= (ntype == kCut ? a.cut_after : 1))
+              split_mask |= 1u << (E - 3);
+            const int ct = child_type(ntype, ndone);
+            if (passed) {
+              P = cO;
+              O = cP;
+            } else {
+              P = cP;
+              O = cO;
+              const int t = alpha;
+              alpha = -beta;
+              beta = -t;
+            }
+            --E;
+            enter_node(cm, ct);
+          }
+        }
+      }
+      if (commit) {
+        best = max(best, v);
+        alpha = max(alpha, best);
+        ndone = min(ndone + 1, 3);
+      }
+    } else if (mode == WAIT) {
+      // Waiting for helpers of the current node; stop them once it is cut.
+      const uint32_t sx = get_sx(E);
+      if (sx_pending(sx) == 0) {
+        mode = SEARCH;
+        commit = true;
+        if (dead_level == E) dead_level = 64;
+      } else if (dead_level != E && max(alpha, sx_best(sx)) >= beta) {
+        dead_level = E;
+        dead_new = true;
+      }
+    } else if (mode == ABORT) {
+      if (helpers_out == 0) {
+        report_aborted = true;
+        mode = REPORT;
+      }
+    }
+
+    // ---- 6. return from every finished node ----
+    if (commit) {
+      for (;;) {
+        if (E >= a.split_min) {
+          const uint32_t sx = get_sx(E);
+          const int pd = sx_pending(sx);
+          best = max(best, sx_best(sx));
+          alpha = max(alpha, best);
+          if (!(alpha >= beta || !moves)) break;
+          if (pd != 0) {
+            mode = WAIT;
+            if (alpha >= beta) {
+              dead_level = E;
+              dead_new = true;
+            }
+            break;
+          }
+        } else if (!(alpha >= beta || !moves)) {
+          break;
+        }
+        if (E == Eroot) {
+          const int r = negate ? -best : best;
+          if (owner < 0) {
+            a.scores[idx] = (int8_t)r;
+            mode = IDLE;
+          } else {
+            report_value = r;
+            report_aborted = false;
+            mode = REPORT;
+          }
+          break;
+        }
+        ++E;
+        uint64_t fb;
+        uint32_t w;
+        if (in_lds(E)) {
+          const int k = lds_k(E);
+          fb = l_fb[k];
+          moves = l_mv[k];
+          order = l_od[k];
+          w = l_wd[k];
+        } else {
+          const size_t k = hbm_k(E);
+          fb = hbm_ld(&a.g_fb[k]);
+          moves = hbm_ld(&a.g_mv[k]);
+          order = hbm_ld(&a.g_od[k]);
+          w = hbm_ld(&a.g_wd[k]);
+        }
+        split_mask &= ~(1u << (E - 3));
+        const int palpha = frame_val(w, 0);
+        const int pbeta = frame_val(w, 7);
+        const int pbest = frame_val(w, 14);
+        undo_move(P, O, fb, w);
+        const int vv = ((w >> 27) & 1) ? best : -best;
+        beta = pbeta;
+        best = max(pbest, vv);
+        alpha = max(palpha, best);
+        ndone = min((int)((w >> 28) & 3) + 1, 3);
+        ntype = (int)(w >> 30);
+        scoring = false;
+      }
     }
   }
 
@@ -262,6 +631,8 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
   atomicAdd(&a.ctr->nodes, c_nodes);
   atomicAdd(&a.ctr->score_steps, c_steps);
   atomicAdd(&a.ctr->iterations, c_iters);
+  atomicAdd(&a.ctr->steals, c_steals);
+  atomicAdd(&a.ctr->lane_slots, c_slots);
 }
 
 __global__ void reset_counters_kernel(SolveCounters* c) {
@@ -271,11 +642,13 @@ __global__ void reset_counters_kernel(SolveCounters* c) {
   c->nodes = 0;
   c->score_steps = 0;
   c->iterations = 0;
+  c->steals = 0;
+  c->lane_slots = 0;
 }
 
 // ---------------------------------------------------------------------------
 // Host side: pick the instantiation whose stack covers DEPTH, size the grid
-// to what stays resident, launch.
+// to what stays resident, lay out the HBM frames, launch.
 
 using KernelFn = void (*)(SolveArgs);
 
@@ -293,19 +666,42 @@ static int stack_levels_for(int max_depth, KernelFn* fn, int ordering) {
   if (d <= 16) { *fn = pick_order<16>(ordering); return 16; }
   if (d <= 20) { *fn = pick_order<20>(ordering); return 20; }
   if (d <= 24) { *fn = pick_order<24>(ordering); return 24; }
-  if (d <= 28) { *fn = pick_order<28>(ordering); return 28; }
   *fn = pick_order<32>(ordering);
   return 32;
 }
 
+Workspace::~Workspace() {
+  if (ctr) (void)hipFree(ctr);
+  if (frames) (void)hipFree(frames);
+}
+
+int Workspace::ensure(size_t frame_bytes) {
+  hipError_t e;
+  if (!ctr && (e = hipMalloc(&ctr, sizeof(SolveCounters))) != hipSuccess)
+    return hip_fail(e, "hipMalloc(counters)");
+  if (frame_bytes > frames_bytes) {
+    if (frames) (void)hipFree(frames);
+    frames = nullptr;
+    frames_bytes = 0;
+    if ((e = hipMalloc(&frames, frame_bytes)) != hipSuccess) return hip_fail(e, "hipMalloc(stack frames)");
+    frames_bytes = frame_bytes;
+  }
+  return GO2_OK;
+}
+
 int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_scores,
-                 const go2_config& cfg, SolveCounters* d_ctr, hipStream_t stream, hipEvent_t ev0,
+                 const go2_config& cfg, Workspace& ws, hipStream_t stream, hipEvent_t ev0,
                  hipEvent_t ev1) {
-  if (n > 0xFFFFFFF0ULL) return fail(GO2_E_ARG, "too many boards in one solve (max 2^32-16)");
+  if (n > 0xFFFFFF00ULL) return fail(GO2_E_ARG, "too many boards in one solve (max 2^32-256)");
   KernelFn fn = nullptr;
   const int maxe = stack_levels_for(max_depth, &fn, cfg.ordering);
   const int bs = cfg.block_size;
-  const size_t lds = (size_t)(maxe - 2) * bs * 28;
+  const int levels = maxe - 2;  // frame levels 3..maxe
+  // Keep the stack in LDS up to 12 empties (two 256-lane blocks per CU);
+  // only nodes nearer the root than that use HBM frames.
+  int lds_levels = cfg.lds_levels > 0 ? cfg.lds_levels : 10;
+  if (lds_levels > levels) lds_levels = levels;
+  const size_t lds = (size_t)lds_levels * bs * 32;
   hipError_t e;
   if (lds > 65536) {
     e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -318,16 +714,65 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   int per_cu = 0;
   if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, bs, lds)) != hipSuccess)
     return hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
-  if (per_cu < 1) return fail(GO2_E_ARG, "block size / stack depth do not fit one CU");
+  if (per_cu < 1) return fail(GO2_E_ARG, "block size / LDS levels do not fit one CU");
   if (cfg.blocks_per_cu > 0 && cfg.blocks_per_cu < per_cu) per_cu = cfg.blocks_per_cu;
+  const size_t waves_per_block = (size_t)bs / 64;
   size_t grid = (size_t)cus * per_cu;
-  const size_t need = (n + bs - 1) / bs;
-  if (grid > need) grid = need;
+  // Never launch more wavefronts than boards.
+  const size_t max_grid = (n + waves_per_block - 1) / waves_per_block;
+  if (grid > max_grid) grid = max_grid;
   if (grid < 1) grid = 1;
+  const size_t waves = grid * waves_per_block;
+  int K = cfg.positions_per_wave;
+  if (K <= 0) K = (int)((n + waves - 1) / waves);
+  if (K > 64) K = 64;
+  if (K < 1) K = 1;
+  int split_min = cfg.split_min_empties > 0 ? cfg.split_min_empties : 8;
+  if (split_min < 5) split_min = 5;
+  if (split_min > maxe + 1) split_min = maxe + 1;  // no split levels at all
 
-  reset_counters_kernel<<<1, 1, 0, stream>>>(d_ctr);
+  const size_t total_lanes = grid * bs;
+  const size_t g_levels = (size_t)(levels - lds_levels);
+  // layout: fb, mv, od (u64) | wd, sx (u32), each [level][total_lanes]
+  const size_t bytes = total_lanes * g_levels * 32 + 5 * 256;
+  int rc = ws.ensure(bytes);
+  if (rc != GO2_OK) return rc;
+  uint8_t* base = reinterpret_cast<uint8_t*>(ws.frames);
+  SolveArgs args{};
+  args.boards = d_boards;
+  args.scores = d_scores;
+  args.n = (unsigned int)n;
+  args.depth = max_depth;
+  args.kff = cfg.mobility_min_empties;
+  args.split_min = split_min;
+  args.cut_after = cfg.split_cut_after > 0 ? cfg.split_cut_after : 3;
+  args.lds_levels = lds_levels;
+  args.K = K;
+  args.total_lanes = (unsigned int)total_lanes;
+  args.limit_ticks = 0;
+  args.ctr = ws.ctr;
+  size_t off = 0;
+  auto take = [&](size_t b) {
+    uint8_t* p = base + off;
+    off += (b + 255) & ~(size_t)255;
+    return p;
+  };
+  args.g_fb = reinterpret_cast<uint64_t*>(take(total_lanes * g_levels * 8));
+  args.g_mv = reinterpret_cast<uint64_t*>(take(total_lanes * g_levels * 8));
+  args.g_od = reinterpret_cast<uint64_t*>(take(total_lanes * g_levels * 8));
+  args.g_wd = reinterpret_cast<uint32_t*>(take(total_lanes * g_levels * 4));
+  args.g_sx = reinterpret_cast<uint32_t*>(take(total_lanes * g_levels * 4));
+  if (off > ws.frames_bytes) {
+    rc = ws.ensure(off);
+    if (rc != GO2_OK) return rc;
+    return launch_solve(d_boards, n, max_depth, d_scores, cfg, ws, stream, ev0, ev1);
+  }
+  // s_memrealtime runs at 100 MHz; each wavefront adds its own start time.
+  if (cfg.time_limit_ms > 0) args.limit_ticks = (unsigned long long)cfg.time_limit_ms * 100000ULL;
+  ws.grid_waves = waves;
+
+  reset_counters_kernel<<<1, 1, 0, stream>>>(ws.ctr);
   if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "reset_counters launch");
-  SolveArgs args{d_boards, d_scores, (unsigned int)n, max_depth, cfg.mobility_min_empties, d_ctr};
   if (ev0 && (e = hipEventRecord(ev0, stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
   hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(bs), lds, stream, args);
   if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "solve kernel launch");
@@ -335,12 +780,13 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   return GO2_OK;
 }
 
-int collect_stats(SolveCounters* d_ctr, size_t n, hipStream_t stream, hipEvent_t ev0, hipEvent_t ev1,
+int collect_stats(Workspace& ws, hipStream_t stream, hipEvent_t ev0, hipEvent_t ev1,
                   go2_stats* stats) {
   SolveCounters h{};
-  hipError_t e = hipMemcpyAsync(&h, d_ctr, sizeof(h), hipMemcpyDeviceToHost, stream);
+  hipError_t e = hipMemcpyAsync(&h, ws.ctr, sizeof(h), hipMemcpyDeviceToHost, stream);
   if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(counters)");
   if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+  if (h.error == kTimedOut) return fail(GO2_E_TIMEOUT, "solve exceeded its time limit");
   if (h.error) {
     return fail(GO2_E_BOARD, "board " + std::to_string(h.error - 1) +
                                  " is invalid (overlapping discs or more empties than DEPTH)");
@@ -351,10 +797,12 @@ int collect_stats(SolveCounters* d_ctr, size_t n, hipStream_t stream, hipEvent_t
     stats->nodes = h.nodes;
     stats->score_steps = h.score_steps;
     stats->iterations = h.iterations;
+    stats->steals = h.steals;
+    stats->lane_slots = h.lane_slots;
+    stats->grid_waves = ws.grid_waves;
     float ms = 0.f;
     if (ev0 && ev1 && hipEventElapsedTime(&ms, ev0, ev1) == hipSuccess) stats->kernel_ms = ms;
   }
-  (void)n;
   return GO2_OK;
 }
 

@@ -30,6 +30,7 @@ extern "C" {
 #define GO2_E_NOMEM -4   /* allocation failure */
 #define GO2_E_STATE -5   /* call out of order (e.g. download before launch) */
 #define GO2_E_PARSE -6   /* malformed base81 text */
+#define GO2_E_TIMEOUT -7 /* a solve ran past config.time_limit_ms and was abandoned */
 
 /* Largest number of empty squares one board may have. */
 #define GO2_MAX_EMPTIES 32
@@ -47,14 +48,23 @@ typedef struct go2_board {
   uint64_t opponent;
 } go2_board;
 
-/* Launch and search knobs.  go2_config_default() fills a valid set. */
+/* Launch and search knobs.  go2_config_default() fills a valid set; zero in
+ * an "auto" field lets the library choose from the batch and the device. */
 typedef struct go2_config {
-  int32_t ordering;            /* GO2_ORDER_* */
+  int32_t ordering;             /* GO2_ORDER_* */
   int32_t mobility_min_empties; /* fastest-first at nodes with >= this many empties */
   int32_t learned_min_empties;  /* evaluator ordering at nodes with >= this many empties */
-  int32_t block_size;          /* lanes per workgroup, multiple of 64, 64..1024 */
-  int32_t blocks_per_cu;       /* resident workgroups per CU to launch; 0 = as many as LDS allows */
-  int32_t reserved[11];
+  int32_t block_size;           /* lanes per workgroup, multiple of 64, 64..1024 */
+  int32_t blocks_per_cu;        /* resident workgroups per CU to launch; 0 = as many as fit */
+  int32_t positions_per_wave;   /* boards a wavefront owns at once (1..64); 0 = auto from n.
+                                   Lanes without a board of their own help the others. */
+  int32_t split_min_empties;    /* idle lanes take sibling subtrees of nodes with >= this many
+                                   empties (young-brothers-wait); 0 = auto, >60 = never */
+  int32_t lds_levels;           /* stack levels (3, 4, ...) kept in LDS, the rest in HBM; 0 = auto */
+  int32_t time_limit_ms;        /* abandon a launch after this long (GO2_E_TIMEOUT); 0 = no limit */
+  int32_t split_cut_after;      /* at nodes expected to cut, helpers wait for this many children
+                                   (1..3) instead of one; 0 = auto */
+  int32_t reserved[6];
 } go2_config;
 
 /* Counters of one solve.  Replaces the reference's `total` node counter
@@ -65,7 +75,10 @@ typedef struct go2_stats {
   uint64_t score_steps; /* child evaluations spent only on move ordering */
   uint64_t iterations;  /* lane-iterations of the search loop (sum over lanes) */
   double kernel_ms;     /* device time of the solve kernel(s), HIP events on the solve stream */
-  uint64_t reserved[8];
+  uint64_t steals;      /* subtrees handed from a busy lane to an idle one */
+  uint64_t lane_slots;  /* lanes launched x loop trips of their wavefront (busy + idle) */
+  uint64_t grid_waves;  /* wavefronts launched */
+  uint64_t reserved[5];
 } go2_stats;
 
 /* ---- library ---------------------------------------------------------------- */

@@ -15,6 +15,7 @@ ORDERINGS = {"static": go2.ORDER_STATIC, "mobility": go2.ORDER_MOBILITY}
 
 
 def cfg(name, **kw):
+    kw.setdefault("time_limit_ms", 100000)
     return go2.default_config(ordering=ORDERINGS[name], **kw)
 
 
@@ -162,3 +163,37 @@ def test_cli_matches_oracle(tmp_path):
     want, _, _ = ol.solve_batch(np.array(uniq, dtype=np.uint64), 10, threads=16)
     expect = "".join(f"{ol.encode(p, o)} {int(s)}\n" for (p, o), s in zip(uniq, want))
     assert out.read_text() == expect
+
+
+# Helper lanes: every combination of ownership, split depth, LDS/HBM stack
+# split and block size must give the same exact scores.
+SPLIT_CASES = [
+    dict(positions_per_wave=1, split_min_empties=5, lds_levels=1, block_size=64),
+    dict(positions_per_wave=2, split_min_empties=6, lds_levels=5, block_size=256),
+    dict(positions_per_wave=8, split_min_empties=8, lds_levels=10, block_size=128),
+    dict(positions_per_wave=64, split_min_empties=99, lds_levels=3, block_size=256),
+    dict(positions_per_wave=4, split_min_empties=5, lds_levels=30, block_size=64),
+]
+
+
+@pytest.mark.parametrize("case", range(len(SPLIT_CASES)))
+@pytest.mark.parametrize("ordering", ["static", "mobility"])
+def test_helpers_keep_scores_exact(case, ordering):
+    b = np.concatenate([ol.generate(606, 2, 300, 12), ol.generate(607, 2, 200, 10),
+                        ol.generate(608, 2, 40, 14)])
+    want, _, _ = ol.solve_batch(b, 14, threads=16)
+    got, st = go2.solve(go2.as_boards(b), 14, cfg(ordering, **SPLIT_CASES[case]))
+    assert np.array_equal(got, want)
+    if SPLIT_CASES[case]["split_min_empties"] < 14 and SPLIT_CASES[case]["positions_per_wave"] < 8:
+        assert st["steals"] > 0
+
+
+def test_helpers_on_a_single_board():
+    # One deep board, one wavefront: 63 helper lanes.
+    b = ol.generate(1, 4, 3, 16)
+    want, _, _ = ol.solve_batch(b, 16, threads=3)
+    for i in range(3):
+        got, st = go2.solve(go2.as_boards(b[i:i + 1]), 16,
+                            cfg("mobility", positions_per_wave=1, split_min_empties=6))
+        assert got[0] == want[i]
+        assert st["steals"] > 0
