@@ -49,7 +49,7 @@ def main() -> int:
     ap.add_argument("--empties", type=int, default=16)
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--ordering", choices=["best", "mobility", "learned", "static"], default="best")
-    ap.add_argument("--plain-n", type=int, default=64, help="positions for the plain-kernel leg")
+    ap.add_argument("--plain-n", type=int, default=512, help="positions for the plain-kernel leg (0 = skip)")
     ap.add_argument("--plain-limit-s", type=int, default=150, help="time limit of the plain leg")
     ap.add_argument("--cpu-sample", type=int, default=192, help="positions for the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
@@ -64,8 +64,16 @@ def main() -> int:
 
     E, n = args.empties, args.n
     order_name = args.ordering
-    if order_name == "best":
-        order_name = "mobility"
+    learned_meta = None
+    if order_name in ("best", "learned"):
+        from gpuothello2_amd import learn
+        if os.path.exists(learn.WEIGHTS_PATH):
+            learned_meta = learn.install()
+            order_name = "learned"
+        elif order_name == "learned":
+            raise SystemExit(f"no trained evaluator at {learn.WEIGHTS_PATH}")
+        else:
+            order_name = "mobility"
     ordering = {"static": go2.ORDER_STATIC, "mobility": go2.ORDER_MOBILITY, "learned": go2.ORDER_LEARNED}[order_name]
     cfg = go2.default_config(ordering=ordering)
 
@@ -110,30 +118,31 @@ def main() -> int:
     plain_value = None
     plain_match = None
     head_small_value = None
-    log(f"[bench] plain leg: {pn} positions, static ordering")
-    plain_cfg = go2.default_config(ordering=go2.ORDER_STATIC, time_limit_ms=args.plain_limit_s * 1000)
-    pt = go2.Task(pn, E, plain_cfg)
-    pt.upload(boards[:pn])
-    try:
-        pt.launch()
-        pt.wait()
-        plain_scores, pst = pt.download()
-        plain_value = pn / (pst["kernel_ms"] / 1e3)
-    except go2.Go2Error as e:
-        log(f"[bench] plain leg did not finish: {e}")
-        plain_scores = None
-    pt.close()
-    ht = go2.Task(pn, E, cfg)
-    ht.upload(boards[:pn])
-    ht.launch()
-    ht.wait()
-    head_scores, hst = ht.download()
-    ht.close()
-    head_small_value = pn / (hst["kernel_ms"] / 1e3)
-    if plain_scores is not None:
-        plain_match = bool(np.array_equal(plain_scores, head_scores) and np.array_equal(head_scores, scores[:pn]))
-        log(f"[bench] plain on {pn}: {plain_value:.1f} pos/s; headline on the same {pn}: "
-            f"{head_small_value:.1f} pos/s; scores match: {plain_match}")
+    if pn > 0:
+        log(f"[bench] plain leg: {pn} positions, static ordering")
+        plain_cfg = go2.default_config(ordering=go2.ORDER_STATIC, time_limit_ms=args.plain_limit_s * 1000)
+        pt = go2.Task(pn, E, plain_cfg)
+        pt.upload(boards[:pn])
+        try:
+            pt.launch()
+            pt.wait()
+            plain_scores, pst = pt.download()
+            plain_value = pn / (pst["kernel_ms"] / 1e3)
+        except go2.Go2Error as e:
+            log(f"[bench] plain leg did not finish: {e}")
+            plain_scores = None
+        pt.close()
+        ht = go2.Task(pn, E, cfg)
+        ht.upload(boards[:pn])
+        ht.launch()
+        ht.wait()
+        head_scores, hst = ht.download()
+        ht.close()
+        head_small_value = pn / (hst["kernel_ms"] / 1e3)
+        if plain_scores is not None:
+            plain_match = bool(np.array_equal(plain_scores, head_scores) and np.array_equal(head_scores, scores[:pn]))
+            log(f"[bench] plain on {pn}: {plain_value:.1f} pos/s; headline on the same {pn}: "
+                f"{head_small_value:.1f} pos/s; scores match: {plain_match}")
 
     # ---- CPU baseline: the reference method on the host cores ----------------------------
     cpu = None
@@ -174,6 +183,9 @@ def main() -> int:
             "ratio_vs_plain_same_sample": (head_small_value / plain_value) if plain_value else None,
             "score_match_plain": plain_match, "score_match_cpu": cpu_match,
             "nodes_per_position": nodes_per_launch / n,
+            "evaluator": None if learned_meta is None else {
+                k: learned_meta[k] for k in ("positions", "empties", "split", "valid_mse",
+                                             "valid_mse_constant_predictor")},
         },
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_INT32_TOPS,
                      "unit": "Tops/s (int32 lane-ops)", "frac": achieved / PEAK_INT32_TOPS,

@@ -40,7 +40,9 @@ EXPORTED_SYMBOLS = (
     "go2_board_from_text", "go2_board_to_text", "go2_solve", "go2_solve_device",
     "go2_task_create", "go2_task_upload", "go2_task_launch", "go2_task_query",
     "go2_task_wait", "go2_task_download", "go2_task_destroy", "go2_generate",
-    "go2_generate_device",
+    "go2_generate_device", "go2_pattern_info", "go2_pattern_features_device",
+    "go2_pattern_eval_device", "go2_sgd_gradient_device", "go2_sgd_apply_device",
+    "go2_quantize_weights_device", "go2_set_evaluator",
 )
 
 BOARD_DTYPE = np.dtype([("player", "<u8"), ("opponent", "<u8")])
@@ -66,7 +68,9 @@ class Config(ctypes.Structure):
         ("lds_levels", ctypes.c_int32),
         ("time_limit_ms", ctypes.c_int32),
         ("split_cut_after", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 6),
+        ("learned_key_eval", ctypes.c_int32),
+        ("learned_key_mobility", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
     ]
 
 
@@ -124,13 +128,18 @@ def lib() -> ctypes.CDLL:
     L.go2_task_destroy.restype = None
     L.go2_generate.argtypes = [u64, u32, sz, i32, i32, vp]
     L.go2_generate_device.argtypes = [u64, u32, sz, i32, i32, vp, vp]
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.go2_pattern_info.argtypes = [ip, ip, ip]
+    L.go2_pattern_features_device.argtypes = [vp, sz, vp, vp, vp]
+    L.go2_pattern_eval_device.argtypes = [vp, vp, vp, sz, vp, vp]
+    L.go2_sgd_gradient_device.argtypes = [vp, vp, vp, vp, vp, sz, vp, vp, vp, vp]
+    L.go2_sgd_apply_device.argtypes = [vp, vp, vp, vp, ctypes.c_float, vp]
+    L.go2_quantize_weights_device.argtypes = [vp, vp, ctypes.c_float, vp, vp, vp]
+    L.go2_set_evaluator.argtypes = [vp, vp, ctypes.c_float]
     for name in EXPORTED_SYMBOLS:
         fn = getattr(L, name)
-        if fn.restype is ctypes.c_int or name in ("go2_solve", "go2_solve_device", "go2_set_device",
-                                                   "go2_board_from_text", "go2_task_upload",
-                                                   "go2_task_launch", "go2_task_query", "go2_task_wait",
-                                                   "go2_task_download", "go2_generate",
-                                                   "go2_generate_device"):
+        if name not in ("go2_last_error", "go2_config_default", "go2_board_to_text",
+                        "go2_task_create", "go2_task_destroy"):
             fn.restype = i32
     _lib = L
     return L
@@ -257,3 +266,18 @@ class Task:
             self.close()
         except Exception:
             pass
+
+
+def pattern_info() -> tuple[int, int, int]:
+    a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    lib().go2_pattern_info(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+    return a.value, b.value, c.value
+
+
+def set_evaluator(w16: np.ndarray, bias16: np.ndarray, scale: float) -> None:
+    """Install the int16 evaluator used by ORDER_LEARNED solves."""
+    _, nw, nb = pattern_info()
+    w = np.ascontiguousarray(w16, dtype=np.int16)
+    b = np.ascontiguousarray(bias16, dtype=np.int16)
+    assert w.shape == (nw,) and b.shape == (nb,)
+    _check(lib().go2_set_evaluator(w.ctypes.data, b.ctypes.data, float(scale)))

@@ -44,6 +44,7 @@
 //   * A wall-clock deadline (s_memrealtime) bounds every launch.
 #include "bitboard.hpp"
 #include "internal.hpp"
+#include "pattern.hpp"
 
 namespace go2 {
 
@@ -130,6 +131,13 @@ struct SolveArgs {
   int K;              // boards owned by one wavefront at a time
   unsigned int total_lanes;
   unsigned long long limit_ticks;  // time limit in s_memrealtime ticks (100 MHz); 0 = none
+  // learned ordering (ORDER 2) at nodes with >= learned_min empties:
+  // key = clamp(((eval(child) * key_eval_mul) >> 12) + replies * key_mob_mul + 32, 0, 63)
+  const int16_t* ev_w;
+  const int16_t* ev_bias;
+  int learned_min;
+  int key_eval_mul;
+  int key_mob_mul;
   SolveCounters* ctr;
   // HBM frames of levels >= lds_levels+3: [level - lds_levels - 3][total_lanes]
   uint64_t* g_fb;
@@ -197,6 +205,18 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
   uint64_t* l_od = l_mv + LL * bs;
   uint32_t* l_wd = reinterpret_cast<uint32_t*>(l_od + LL * bs);
   uint32_t* l_sx = l_wd + LL * bs;
+  uint16_t* t3 = reinterpret_cast<uint16_t*>(l_sx + LL * bs);  // base-3 table (ORDER 2)
+  if constexpr (ORDER == 2) {
+    for (int b = tid; b < kBase3Size; b += bs) {
+      uint32_t v = 0, p = 1;
+      for (int k = 0; k < 10; ++k) {
+        v += ((b >> k) & 1) * p;
+        p *= 3;
+      }
+      t3[b] = (uint16_t)v;
+    }
+    __syncthreads();
+  }
 
   // Every frame slot is read and written only by the lane that owns it.
   // LDS levels use ds_* accesses and HBM levels global ones (no flat
@@ -493,8 +513,16 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
 
       if (scoring) {
         // Fastest first: fewest replies, corners preferred, X-squares delayed.
+        // Near the root the learned evaluator of the child joins the key.
         ++c_steps;
-        const int k = min(popc64(cm), 15) * 4 + ((bit & kCorners) ? 0 : 2) + ((bit & kXSquares) ? 1 : 0);
+        int k = min(popc64(cm), 15) * 4 + ((bit & kCorners) ? 0 : 2) + ((bit & kXSquares) ? 1 : 0);
+        if constexpr (ORDER == 2) {
+          if (E >= a.learned_min) {
+            int ev = a.ev_bias[E - 1];
+            for_each_feature(cP, cO, t3, [&](int, uint32_t fi) { ev += a.ev_w[fi]; });
+            k = min(max(((ev * a.key_eval_mul) >> 12) + popc64(cm) * a.key_mob_mul + 32, 0), 63);
+          }
+        }
         const uint64_t valid = (1ULL << (7 * nkeys)) - 1;
         const uint64_t le = ((((uint64_t)k * kLo7) | kHi7) - keys) & kHi7 & valid;
         const int c = popc64(le);  // entries with key <= k: insert after them (stable)
@@ -656,9 +684,42 @@ template <int MAXE>
 static KernelFn pick_order(int ordering) {
   switch (ordering) {
     case GO2_ORDER_STATIC: return solve_kernel<MAXE, 0>;
-    default: return solve_kernel<MAXE, 1>;
+    case GO2_ORDER_MOBILITY: return solve_kernel<MAXE, 1>;
+    default: return solve_kernel<MAXE, 2>;
   }
 }
+
+// Installed int16 evaluator per device (GO2_ORDER_LEARNED).
+struct EvalTables {
+  int16_t* w = nullptr;
+  int16_t* bias = nullptr;
+  float scale = 0.f;
+};
+static EvalTables g_eval[64];
+
+}  // namespace go2
+
+extern "C" int go2_set_evaluator(const int16_t* w, const int16_t* bias, float scale) {
+  using namespace go2;
+  if (!w || !bias || !(scale > 0.f)) return fail(GO2_E_ARG, "null weights or non-positive scale");
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev < 0 || dev >= 64) return fail(GO2_E_ARG, "device index out of range");
+  EvalTables& t = g_eval[dev];
+  if (!t.w && (e = hipMalloc(&t.w, kPatWeights * sizeof(int16_t))) != hipSuccess)
+    return hip_fail(e, "hipMalloc(evaluator)");
+  if (!t.bias && (e = hipMalloc(&t.bias, kPatBias * sizeof(int16_t))) != hipSuccess)
+    return hip_fail(e, "hipMalloc(evaluator bias)");
+  if ((e = hipMemcpy(t.w, w, kPatWeights * sizeof(int16_t), hipMemcpyHostToDevice)) != hipSuccess)
+    return hip_fail(e, "hipMemcpy(evaluator)");
+  if ((e = hipMemcpy(t.bias, bias, kPatBias * sizeof(int16_t), hipMemcpyHostToDevice)) != hipSuccess)
+    return hip_fail(e, "hipMemcpy(evaluator bias)");
+  t.scale = scale;
+  return GO2_OK;
+}
+
+namespace go2 {
 
 static int stack_levels_for(int max_depth, KernelFn* fn, int ordering) {
   const int d = max_depth < 3 ? 3 : max_depth;
@@ -701,7 +762,7 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   // only nodes nearer the root than that use HBM frames.
   int lds_levels = cfg.lds_levels > 0 ? cfg.lds_levels : 10;
   if (lds_levels > levels) lds_levels = levels;
-  const size_t lds = (size_t)lds_levels * bs * 32;
+  const size_t lds = (size_t)lds_levels * bs * 32 + (cfg.ordering == GO2_ORDER_LEARNED ? kBase3Size * 2 : 0);
   hipError_t e;
   if (lds > 65536) {
     e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -750,6 +811,18 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   args.K = K;
   args.total_lanes = (unsigned int)total_lanes;
   args.limit_ticks = 0;
+  if (cfg.ordering == GO2_ORDER_LEARNED) {
+    const EvalTables& t = g_eval[dev];
+    if (!t.w) return fail(GO2_E_STATE, "GO2_ORDER_LEARNED needs go2_set_evaluator() on this device");
+    args.ev_w = t.w;
+    args.ev_bias = t.bias;
+    args.learned_min = cfg.learned_min_empties;
+    // key units: by default one key step per four discs of the evaluator
+    // (`scale` units per disc) and one per reply, the best of a sweep over
+    // 16-empty positions with the 14-empty-trained table.
+    args.key_eval_mul = cfg.learned_key_eval > 0 ? cfg.learned_key_eval : (int)(1024.f / t.scale);
+    args.key_mob_mul = cfg.learned_key_mobility > 0 ? cfg.learned_key_mobility : 1;
+  }
   args.ctr = ws.ctr;
   size_t off = 0;
   auto take = [&](size_t b) {

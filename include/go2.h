@@ -64,7 +64,10 @@ typedef struct go2_config {
   int32_t time_limit_ms;        /* abandon a launch after this long (GO2_E_TIMEOUT); 0 = no limit */
   int32_t split_cut_after;      /* at nodes expected to cut, helpers wait for this many children
                                    (1..3) instead of one; 0 = auto */
-  int32_t reserved[6];
+  int32_t learned_key_eval;     /* GO2_ORDER_LEARNED move key = eval(child)*this/4096 (evaluator
+                                   units) + replies*learned_key_mobility; 0 = auto */
+  int32_t learned_key_mobility;
+  int32_t reserved[4];
 } go2_config;
 
 /* Counters of one solve.  Replaces the reference's `total` node counter
@@ -145,6 +148,42 @@ int go2_generate(uint64_t seed, uint32_t config_id, size_t n, int empties_min, i
 /* Same, writing device memory. */
 int go2_generate_device(uint64_t seed, uint32_t config_id, size_t n, int empties_min,
                         int empties_max, go2_board *d_out, void *hip_stream);
+
+/* ---- learned move ordering: pattern evaluator ------------------------------------
+ * A linear model over 46 ternary pattern instances (11 classes, int16 weight
+ * per class configuration) plus one bias per number of empties, predicting
+ * the exact disc difference for the side to move.  Replaces the reference's
+ * float Evaluator (reference eval.cuh:5-17, eval.cu:12-142), whose weight
+ * file it does not ship; here the model is trained by the kernels below on
+ * scores the solver produces, and orders moves at shallow nodes of
+ * GO2_ORDER_LEARNED solves (scores never depend on it). */
+/* Sizes: instances per board (46), weights (167265), bias terms (64). */
+int go2_pattern_info(int *n_instances, int *n_weights, int *n_bias);
+/* Global weight index of every instance: d_feat[n*46] (uint32), and the
+ * number of empties of every board: d_empties[n]. */
+int go2_pattern_features_device(const go2_board *d_boards, size_t n, uint32_t *d_feat,
+                                uint8_t *d_empties, void *hip_stream);
+/* int32 evaluation sum_k w[f_k] + bias[empties] of every board. */
+int go2_pattern_eval_device(const int16_t *d_w, const int16_t *d_bias, const go2_board *d_boards,
+                            size_t n, int32_t *d_out, void *hip_stream);
+/* SGD forward + gradient of the batch squared error against d_y (FP32 master
+ * weights): adds 2*(pred-y) per occurrence into d_grad[n_weights+n_bias],
+ * occurrence counts into d_count, and the sum of squared residuals into
+ * *d_sse.  d_grad/d_count must start zeroed (go2_sgd_apply_device leaves them
+ * zeroed). */
+int go2_sgd_gradient_device(const float *d_w, const float *d_bias, const uint32_t *d_feat,
+                            const uint8_t *d_empties, const float *d_y, size_t n, float *d_grad,
+                            uint32_t *d_count, double *d_sse, void *hip_stream);
+/* w -= lr * grad / count for every weight with count > 0; clears grad and count. */
+int go2_sgd_apply_device(float *d_w, float *d_bias, float *d_grad, uint32_t *d_count, float lr,
+                         void *hip_stream);
+/* int16 copy of the model: round(w * scale), clamped to +-32767. */
+int go2_quantize_weights_device(const float *d_w, const float *d_bias, float scale, int16_t *d_w16,
+                                int16_t *d_bias16, void *hip_stream);
+/* Install the int16 evaluator (host arrays of n_weights and n_bias entries;
+ * `scale` = units per disc) used by GO2_ORDER_LEARNED solves on the current
+ * device.  Without one, GO2_ORDER_LEARNED solves fail with GO2_E_STATE. */
+int go2_set_evaluator(const int16_t *w, const int16_t *bias, float scale);
 
 #ifdef __cplusplus
 }
