@@ -70,7 +70,8 @@ class Config(ctypes.Structure):
         ("split_cut_after", ctypes.c_int32),
         ("learned_key_eval", ctypes.c_int32),
         ("learned_key_mobility", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 4),
+        ("max_pops_per_trip", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 3),
     ]
 
 

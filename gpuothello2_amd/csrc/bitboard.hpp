@@ -110,4 +110,86 @@ GO2_HD int last_score(uint64_t P, uint64_t O, int sq) {
   return final_score(P, O);
 }
 
+
+// ---- last empty square by table ---------------------------------------------
+// On a board whose only empty square is sq, the discs a side flips there
+// depend, line by line, only on that side's discs along the line.  kLastCount
+// [pos][p] = discs flipped along one 8-square line when the mover's discs on
+// it form the bit pattern p and it plays at index pos; squares off the board
+// are 0 in p and (being opponent-like but never followed by a mover disc)
+// flip nothing.
+GO2_HD int line_flip_count(int pos, int p) {
+  int n = 0, run = 0;
+  for (int i = pos + 1; i < 8; ++i) {
+    if ((p >> i) & 1) {
+      n += run;
+      break;
+    }
+    ++run;
+  }
+  run = 0;
+  for (int i = pos - 1; i >= 0; --i) {
+    if ((p >> i) & 1) {
+      n += run;
+      break;
+    }
+    ++run;
+  }
+  return n;
+}
+
+// Gather column c of x into bits 0..7 (row order).
+GO2_HD uint32_t column_bits(uint64_t x, int c) {
+  uint64_t t = (x >> c) & 0x0101010101010101ULL;
+  t |= t >> 7;
+  t |= t >> 14;
+  t |= t >> 28;
+  return (uint32_t)(t & 0xFF);
+}
+
+// The four line patterns through sq of a bitboard, each in bits 0..7 (row
+// and diagonals indexed by column, the column indexed by row), plus the
+// patterns of which line squares exist on the board.
+struct LinePatterns {
+  uint32_t row, col, diag, anti;      // mover's discs
+  uint32_t diag_on, anti_on;          // squares of the diagonals on the board
+};
+
+GO2_HD LinePatterns line_patterns(uint64_t x, int sq) {
+  const int r = sq >> 3, c = sq & 7;
+  const int d = c - r, a = c + r - 7;
+  const uint64_t dm = d >= 0 ? 0x8040201008040201ULL >> (8 * d) : 0x8040201008040201ULL << (-8 * d);
+  const uint64_t am = a >= 0 ? 0x0102040810204080ULL << (8 * a) : 0x0102040810204080ULL >> (-8 * a);
+  auto fold = [](uint64_t y) {
+    y |= y >> 32;
+    y |= y >> 16;
+    y |= y >> 8;
+    return (uint32_t)(y & 0xFF);
+  };
+  LinePatterns L;
+  L.row = (uint32_t)((x >> (sq & 56)) & 0xFF);
+  L.col = column_bits(x, c);
+  L.diag = fold(x & dm);
+  L.anti = fold(x & am);
+  L.diag_on = fold(dm);
+  L.anti_on = fold(am);
+  return L;
+}
+
+// Exact score, from P's view, of a board with exactly one empty square sq,
+// using the kLastCount table `t` (8 x 256 bytes).
+template <class T>
+GO2_HD int last_score_tab(uint64_t P, uint64_t O, int sq, const T& t) {
+  const int r = sq >> 3, c = sq & 7;
+  const LinePatterns L = line_patterns(P, sq);
+  const int n1 = t[c * 256 + L.row] + t[r * 256 + L.col] + t[c * 256 + L.diag] + t[c * 256 + L.anti];
+  const int p = popc64(P);
+  if (n1) return 2 * (p + n1) + 2 - 64;
+  const uint32_t hole = ~(1u << c) & 0xFF, holer = ~(1u << r) & 0xFF;
+  const int n2 = t[c * 256 + (~L.row & hole)] + t[r * 256 + (~L.col & holer)] +
+                 t[c * 256 + (~L.diag & L.diag_on & hole)] + t[c * 256 + (~L.anti & L.anti_on & hole)];
+  if (n2) return 2 * (p - n2) - 64;
+  return final_score(P, O);
+}
+
 }  // namespace go2

@@ -138,6 +138,7 @@ struct SolveArgs {
   int learned_min;
   int key_eval_mul;
   int key_mob_mul;
+  int max_pops;       // finished nodes a lane returns from per loop trip
   SolveCounters* ctr;
   // HBM frames of levels >= lds_levels+3: [level - lds_levels - 3][total_lanes]
   uint64_t* g_fb;
@@ -205,7 +206,10 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
   uint64_t* l_od = l_mv + LL * bs;
   uint32_t* l_wd = reinterpret_cast<uint32_t*>(l_od + LL * bs);
   uint32_t* l_sx = l_wd + LL * bs;
-  uint16_t* t3 = reinterpret_cast<uint16_t*>(l_sx + LL * bs);  // base-3 table (ORDER 2)
+  uint8_t* lastc = reinterpret_cast<uint8_t*>(l_sx + LL * bs);  // last-square flip counts
+  for (int k = tid; k < 2048; k += bs) lastc[k] = (uint8_t)line_flip_count(k >> 8, k & 255);
+  uint16_t* t3 = reinterpret_cast<uint16_t*>(lastc + 2048);  // base-3 table (ORDER 2)
+  if constexpr (ORDER != 2) __syncthreads();
   if constexpr (ORDER == 2) {
     for (int b = tid; b < kBase3Size; b += bs) {
       uint32_t v = 0, p = 1;
@@ -267,6 +271,10 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
   uint64_t P = 0, O = 0, moves = 0, order = 0, to_score = 0, keys = 0;
   int alpha = 0, beta = 0, best = 0, E = 0, Eroot = 0, nkeys = 0;
   bool scoring = false, negate = false;
+  bool pchk = false;            // a child without moves: decide pass / game over next trip
+  int pchk_sq = 0;
+  uint64_t pchk_f = 0;
+  bool pop_pending = false;     // more finished nodes to return from next trip
   int ndone = 0, ntype = kPV;   // finished children (saturating at 3) and expected type
   int owner = -1, owner_E = 0;  // owner lane and frame level of a helper's task (-1: own board)
   unsigned int idx = 0;
@@ -290,6 +298,8 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
     nkeys = 0;
     ndone = 0;
     ntype = type;
+    pchk = false;
+    pop_pending = false;
     if (E >= a.split_min) set_sx(E, 0u);
   };
 
@@ -486,32 +496,55 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
     }
 
     // ---- 5. one search step ----
+    // Every searching lane runs one flip and one mobility per trip.  A child
+    // whose side has no move is settled next trip with the other side's
+    // mobility in the same slot (pass or game over), so no trip pays for a
+    // second mobility.
     c_slots += 1;
     bool commit = false;
     int v = 0;
-    if (mode == SEARCH) {
+    if (mode == SEARCH && !pop_pending) {
       ++c_iters;
       int sq;
-      if (scoring) {
-        sq = ctz64(to_score);
-      } else if (order) {
-        sq = (int)(order & 127) - 1;
-        order >>= 7;
+      uint64_t f, mx, my;
+      bool leaf = false;
+      if (pchk) {
+        sq = pchk_sq;
+        f = pchk_f;
       } else {
-        sq = pick_static<ORDER>(moves, ~(P | O));
+        if (scoring) {
+          sq = ctz64(to_score);
+        } else if (order) {
+          sq = (int)(order & 127) - 1;
+          order >>= 7;
+        } else {
+          sq = pick_static<ORDER>(moves, ~(P | O));
+        }
+        f = flip(P, O, sq);
+        leaf = !scoring && E == 2;
       }
       const uint64_t bit = 1ULL << sq;
-      const uint64_t f = flip(P, O, sq);
       const uint64_t cP = O ^ f, cO = P ^ f ^ bit;
-      const bool leaf = !scoring && E == 2;
+      mx = pchk ? cO : cP;
+      my = pchk ? cP : cO;
       uint64_t cm = 0;
       if (leaf) {
-        v = -last_score(cP, cO, ctz64(~(cP | cO)));
+        v = -last_score_tab(cP, cO, ctz64(~(cP | cO)), lastc);
       } else {
-        cm = mobility(cP, cO);
+        cm = mobility(mx, my);
       }
 
-      if (scoring) {
+      bool push = false, passed = false;
+      if (pchk) {
+        pchk = false;
+        if (!cm) {
+          v = -final_score(cP, cO);  // neither side can move: game over
+          commit = true;
+        } else {
+          push = true;
+          passed = true;
+        }
+      } else if (scoring) {
         // Fastest first: fewest replies, corners preferred, X-squares delayed.
         // Near the root the learned evaluator of the child joins the key.
         ++c_steps;
@@ -535,44 +568,43 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
       } else {
         moves ^= bit;
         ++c_nodes;
-        commit = leaf;
-        if (!leaf) {
-          bool passed = false;
-          if (!cm) {
-            const uint64_t om = mobility(cO, cP);
-            if (!om) {
-              v = -final_score(cP, cO);
-              commit = true;
-            } else {
-              cm = om;
-              passed = true;
-            }
-          }
-          if (!commit) {
-            push_frame(E, f | bit, moves, order, pack_frame(alpha, beta, best, sq, passed, ndone, ntype));
-            if (E >= a.split_min && moves && ndone >= (ntype == kCut ? a.cut_after : 1))
-              split_mask |= 1u << (E - 3);
-            const int ct = child_type(ntype, ndone);
-            if (passed) {
-              P = cO;
-              O = cP;
-            } else {
-              P = cP;
-              O = cO;
-              const int t = alpha;
-              alpha = -beta;
-              beta = -t;
-            }
-            --E;
-            enter_node(cm, ct);
-          }
+        if (leaf) {
+          commit = true;
+        } else if (!cm) {
+          pchk = true;
+          pchk_sq = sq;
+          pchk_f = f;
+        } else {
+          push = true;
         }
+      }
+      if (push) {
+        push_frame(E, f | bit, moves, order, pack_frame(alpha, beta, best, sq, passed, ndone, ntype));
+        if (E >= a.split_min && moves && ndone >= (ntype == kCut ? a.cut_after : 1))
+          split_mask |= 1u << (E - 3);
+        const int ct = child_type(ntype, ndone);
+        if (passed) {
+          P = cO;
+          O = cP;
+        } else {
+          P = cP;
+          O = cO;
+          const int t = alpha;
+          alpha = -beta;
+          beta = -t;
+        }
+        --E;
+        enter_node(cm, ct);
       }
       if (commit) {
         best = max(best, v);
         alpha = max(alpha, best);
         ndone = min(ndone + 1, 3);
       }
+    } else if (mode == SEARCH) {
+      // continue returning from finished nodes (pops are capped per trip)
+      pop_pending = false;
+      commit = true;
     } else if (mode == WAIT) {
       // Waiting for helpers of the current node; stop them once it is cut.
       const uint32_t sx = get_sx(E);
@@ -591,9 +623,9 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
       }
     }
 
-    // ---- 6. return from every finished node ----
+    // ---- 6. return from finished nodes, at most max_pops levels per trip ----
     if (commit) {
-      for (;;) {
+      for (int pops = 0;; ++pops) {
         if (E >= a.split_min) {
           const uint32_t sx = get_sx(E);
           const int pd = sx_pending(sx);
@@ -621,6 +653,10 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
             report_aborted = false;
             mode = REPORT;
           }
+          break;
+        }
+        if (pops == a.max_pops) {
+          pop_pending = true;
           break;
         }
         ++E;
@@ -762,7 +798,7 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   // only nodes nearer the root than that use HBM frames.
   int lds_levels = cfg.lds_levels > 0 ? cfg.lds_levels : 10;
   if (lds_levels > levels) lds_levels = levels;
-  const size_t lds = (size_t)lds_levels * bs * 32 + (cfg.ordering == GO2_ORDER_LEARNED ? kBase3Size * 2 : 0);
+  const size_t lds = (size_t)lds_levels * bs * 32 + 2048 + (cfg.ordering == GO2_ORDER_LEARNED ? kBase3Size * 2 : 0);
   hipError_t e;
   if (lds > 65536) {
     e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -811,6 +847,7 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   args.K = K;
   args.total_lanes = (unsigned int)total_lanes;
   args.limit_ticks = 0;
+  args.max_pops = cfg.max_pops_per_trip > 0 ? cfg.max_pops_per_trip : 2;
   if (cfg.ordering == GO2_ORDER_LEARNED) {
     const EvalTables& t = g_eval[dev];
     if (!t.w) return fail(GO2_E_STATE, "GO2_ORDER_LEARNED needs go2_set_evaluator() on this device");

@@ -67,7 +67,8 @@ typedef struct go2_config {
   int32_t learned_key_eval;     /* GO2_ORDER_LEARNED move key = eval(child)*this/4096 (evaluator
                                    units) + replies*learned_key_mobility; 0 = auto */
   int32_t learned_key_mobility;
-  int32_t reserved[4];
+  int32_t max_pops_per_trip;    /* finished nodes a lane returns from per loop trip; 0 = auto (2) */
+  int32_t reserved[3];
 } go2_config;
 
 /* Counters of one solve.  Replaces the reference's `total` node counter
