@@ -794,9 +794,10 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   const int maxe = stack_levels_for(max_depth, &fn, cfg.ordering);
   const int bs = cfg.block_size;
   const int levels = maxe - 2;  // frame levels 3..maxe
-  // Keep the stack in LDS up to 12 empties (two 256-lane blocks per CU);
-  // only nodes nearer the root than that use HBM frames.
-  int lds_levels = cfg.lds_levels > 0 ? cfg.lds_levels : 10;
+  // Keep the stack in LDS up to 11 empties (two 256-lane blocks per CU with
+  // the last-square and base-3 tables); only nodes nearer the root than that
+  // use HBM frames.
+  int lds_levels = cfg.lds_levels > 0 ? cfg.lds_levels : 9;
   if (lds_levels > levels) lds_levels = levels;
   const size_t lds = (size_t)lds_levels * bs * 32 + 2048 + (cfg.ordering == GO2_ORDER_LEARNED ? kBase3Size * 2 : 0);
   hipError_t e;
