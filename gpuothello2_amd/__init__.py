@@ -71,7 +71,9 @@ class Config(ctypes.Structure):
         ("learned_key_eval", ctypes.c_int32),
         ("learned_key_mobility", ctypes.c_int32),
         ("max_pops_per_trip", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 3),
+        ("tt_log2_buckets", ctypes.c_int32),
+        ("tt_min_empties", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 1),
     ]
 
 
@@ -85,7 +87,11 @@ class Stats(ctypes.Structure):
         ("steals", ctypes.c_uint64),
         ("lane_slots", ctypes.c_uint64),
         ("grid_waves", ctypes.c_uint64),
-        ("reserved", ctypes.c_uint64 * 5),
+        ("tt_probes", ctypes.c_uint64),
+        ("tt_hits", ctypes.c_uint64),
+        ("tt_cuts", ctypes.c_uint64),
+        ("tt_stores", ctypes.c_uint64),
+        ("reserved", ctypes.c_uint64 * 1),
     ]
 
     def as_dict(self) -> dict:

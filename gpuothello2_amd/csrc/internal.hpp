@@ -20,6 +20,10 @@ struct SolveCounters {
   unsigned long long iterations;
   unsigned long long steals;
   unsigned long long lane_slots;
+  unsigned long long tt_probes;
+  unsigned long long tt_hits;
+  unsigned long long tt_cuts;
+  unsigned long long tt_stores;
 };
 
 constexpr unsigned int kTimedOut = 0xFFFFFFFFu;
@@ -38,11 +42,17 @@ struct Workspace {
   void* frames = nullptr;
   size_t frames_bytes = 0;
   unsigned long long grid_waves = 0;  // of the last launch
+  // transposition table: 2^tt_log2 buckets of 16 bytes; entries of older
+  // launches are ignored through an 8-bit generation
+  void* tt = nullptr;
+  int tt_log2 = 0;
+  unsigned int tt_gen = 0;
   Workspace() = default;
   Workspace(const Workspace&) = delete;
   Workspace& operator=(const Workspace&) = delete;
   ~Workspace();
   int ensure(size_t frame_bytes);
+  int ensure_tt(int log2_buckets, hipStream_t stream);
 };
 
 // Enqueue counter reset + solve kernel on `stream`; events bracket the solve

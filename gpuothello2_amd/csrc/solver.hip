@@ -139,6 +139,13 @@ struct SolveArgs {
   int key_eval_mul;
   int key_mob_mul;
   int max_pops;       // finished nodes a lane returns from per loop trip
+  // transposition table (null = none): probed when pushing children with
+  // >= tt_min empties, written when such nodes finish
+  ulonglong2* tt;
+  unsigned long long tt_mask;
+  int tt_min;
+  unsigned int tt_gen;
+  const uint64_t* zob;  // Zobrist keys per (byte position 0..15, byte value)
   SolveCounters* ctr;
   // HBM frames of levels >= lds_levels+3: [level - lds_levels - 3][total_lanes]
   uint64_t* g_fb;
@@ -159,6 +166,26 @@ __device__ __forceinline__ int sx_best(uint32_t x) {
 __device__ __forceinline__ uint32_t sx_with_best(uint32_t x, int v) {
   const uint32_t b = (uint32_t)(v / 2 + 64);
   return ((x >> 8) & 255u) >= b ? x : ((x & ~0xFF00u) | (b << 8));
+}
+// Bits 16-22 of the split word: the node's alpha at entry, halved (for the
+// bound type of its transposition-table entry).
+__device__ __forceinline__ uint32_t sx_init(int alpha0) { return ((uint32_t)(alpha0 >> 1) & 127u) << 16; }
+__device__ __forceinline__ int sx_alpha0(uint32_t x) { return ((int)(x << 9) >> 25) * 2; }
+
+// ---- transposition table ------------------------------------------------------
+// Key: Zobrist hashing over the 16 bytes of the two bitboards (one random
+// 64-bit word per byte position and value).  Bucket: (key ^ data, data), so a
+// torn or foreign entry fails the key check (lock-free, no CAS).  Data:
+// lower + 64 (bits 0-7), upper + 64 (8-15), best move (16-21, 63 = none),
+// generation (24-31), empties (32-37).
+__device__ __forceinline__ uint64_t zobrist(uint64_t P, uint64_t O, const uint64_t* z) {
+  uint64_t h = 0;
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    h ^= z[b * 256 + (int)((P >> (8 * b)) & 255)];
+    h ^= z[(8 + b) * 256 + (int)((O >> (8 * b)) & 255)];
+  }
+  return h;
 }
 
 __device__ __forceinline__ uint64_t readlane64(uint64_t x, int lane) {
@@ -277,6 +304,8 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
   bool pop_pending = false;     // more finished nodes to return from next trip
   int ndone = 0, ntype = kPV;   // finished children (saturating at 3) and expected type
   int owner = -1, owner_E = 0;  // owner lane and frame level of a helper's task (-1: own board)
+  int alpha0 = 0;               // current node's alpha at entry
+  int bestmv = 63;              // move that gave the current node's best value (63 = none)
   unsigned int idx = 0;
   uint32_t split_mask = 0;      // bit E-3: this lane's frame at level E offers siblings
   int helpers_out = 0;          // helpers still working on this lane's frames
@@ -286,6 +315,7 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
   bool report_aborted = false;
   bool queue_done = false;      // wavefront-uniform
   unsigned long long c_pos = 0, c_nodes = 0, c_steps = 0, c_iters = 0, c_steals = 0, c_slots = 0;
+  unsigned long long c_probes = 0, c_hits = 0, c_cuts = 0, c_stores = 0;
 
   // A node at level E with legal moves `cm` becomes the current node.
   auto enter_node = [&](uint64_t cm, int type) {
@@ -300,7 +330,9 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
     ntype = type;
     pchk = false;
     pop_pending = false;
-    if (E >= a.split_min) set_sx(E, 0u);
+    alpha0 = alpha;
+    bestmv = 63;
+    if (E >= a.split_min) set_sx(E, sx_init(alpha));
   };
 
   unsigned int trip = 0;
@@ -578,25 +610,56 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
           push = true;
         }
       }
+      int ca = passed ? alpha : -beta, cb = passed ? beta : -alpha;  // child window
+      if (push && !passed && a.tt && E - 1 >= a.tt_min) {
+        // Probe the child: an entry may settle it or narrow its window.
+        ++c_probes;
+        const uint64_t h = zobrist(cP, cO, a.zob);
+        const ulonglong2 b = a.tt[h & a.tt_mask];
+        if ((b.x ^ b.y) == h && (uint32_t)((b.y >> 24) & 255) == a.tt_gen) {
+          ++c_hits;
+          const int lo = (int)(b.y & 255) - 64, up = (int)((b.y >> 8) & 255) - 64;
+          int tv = 0;
+          bool cut = true;
+          if (lo >= cb) {
+            tv = lo;
+          } else if (up <= ca) {
+            tv = up;
+          } else if (lo == up) {
+            tv = lo;
+          } else {
+            cut = false;
+            ca = max(ca, lo);
+            cb = min(cb, up);
+          }
+          if (cut) {
+            ++c_cuts;
+            push = false;
+            v = -tv;
+            commit = true;
+          }
+        }
+      }
       if (push) {
         push_frame(E, f | bit, moves, order, pack_frame(alpha, beta, best, sq, passed, ndone, ntype));
         if (E >= a.split_min && moves && ndone >= (ntype == kCut ? a.cut_after : 1))
           split_mask |= 1u << (E - 3);
         const int ct = child_type(ntype, ndone);
+        if (a.tt && E >= a.tt_min) set_sx(E, (get_sx(E) & ~(63u << 23)) | ((uint32_t)bestmv << 23));
         if (passed) {
           P = cO;
           O = cP;
         } else {
           P = cP;
           O = cO;
-          const int t = alpha;
-          alpha = -beta;
-          beta = -t;
         }
+        alpha = ca;
+        beta = cb;
         --E;
         enter_node(cm, ct);
       }
       if (commit) {
+        if (v > best) bestmv = sq;
         best = max(best, v);
         alpha = max(alpha, best);
         ndone = min(ndone + 1, 3);
@@ -643,6 +706,16 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
         } else if (!(alpha >= beta || !moves)) {
           break;
         }
+        if (a.tt && E >= a.tt_min) {
+          // Store the finished node: fail low -> upper bound, fail high ->
+          // lower bound, inside the window -> exact.
+          ++c_stores;
+          const int lo = best > alpha0 ? best : -64, up = best < beta ? best : 64;
+          const uint64_t d = (uint64_t)(lo + 64) | ((uint64_t)(up + 64) << 8) | ((uint64_t)bestmv << 16) |
+                             ((uint64_t)a.tt_gen << 24) | ((uint64_t)E << 32);
+          const uint64_t h = zobrist(P, O, a.zob);
+          a.tt[h & a.tt_mask] = make_ulonglong2(h ^ d, d);
+        }
         if (E == Eroot) {
           const int r = negate ? -best : best;
           if (owner < 0) {
@@ -684,6 +757,11 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
         beta = pbeta;
         best = max(pbest, vv);
         alpha = max(palpha, best);
+        if (E >= a.split_min) {
+          const uint32_t psx = get_sx(E);
+          alpha0 = sx_alpha0(psx);
+          bestmv = vv > pbest ? (int)((w >> 21) & 63) : (int)((psx >> 23) & 63);
+        }
         ndone = min((int)((w >> 28) & 3) + 1, 3);
         ntype = (int)(w >> 30);
         scoring = false;
@@ -697,6 +775,12 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
   atomicAdd(&a.ctr->iterations, c_iters);
   atomicAdd(&a.ctr->steals, c_steals);
   atomicAdd(&a.ctr->lane_slots, c_slots);
+  if (a.tt) {
+    atomicAdd(&a.ctr->tt_probes, c_probes);
+    atomicAdd(&a.ctr->tt_hits, c_hits);
+    atomicAdd(&a.ctr->tt_cuts, c_cuts);
+    atomicAdd(&a.ctr->tt_stores, c_stores);
+  }
 }
 
 __global__ void reset_counters_kernel(SolveCounters* c) {
@@ -708,6 +792,10 @@ __global__ void reset_counters_kernel(SolveCounters* c) {
   c->iterations = 0;
   c->steals = 0;
   c->lane_slots = 0;
+  c->tt_probes = 0;
+  c->tt_hits = 0;
+  c->tt_cuts = 0;
+  c->tt_stores = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -770,6 +858,57 @@ static int stack_levels_for(int max_depth, KernelFn* fn, int ordering) {
 Workspace::~Workspace() {
   if (ctr) (void)hipFree(ctr);
   if (frames) (void)hipFree(frames);
+  if (tt) (void)hipFree(tt);
+}
+
+__global__ void clear_tt_kernel(ulonglong2* tt, unsigned long long n) {
+  for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (unsigned long long)gridDim.x * blockDim.x)
+    tt[i] = make_ulonglong2(0, 0);
+}
+
+int Workspace::ensure_tt(int log2_buckets, hipStream_t stream) {
+  hipError_t e;
+  if (log2_buckets != tt_log2) {
+    if (tt) (void)hipFree(tt);
+    tt = nullptr;
+    tt_log2 = 0;
+    if ((e = hipMalloc(&tt, (size_t)16 << log2_buckets)) != hipSuccess)
+      return hip_fail(e, "hipMalloc(transposition table)");
+    tt_log2 = log2_buckets;
+    tt_gen = 0;
+  }
+  // generations 1..255; a wrap clears the table (a fill kernel, no memset node)
+  if (++tt_gen > 255 || tt_gen == 1) {
+    tt_gen = 1;
+    const unsigned long long n = 1ULL << tt_log2;
+    hipLaunchKernelGGL(clear_tt_kernel, dim3(4096), dim3(256), 0, stream, (ulonglong2*)tt, n);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "clear table launch");
+  }
+  return GO2_OK;
+}
+
+// Zobrist keys per device: 16 byte positions x 256 byte values, SplitMix64.
+static uint64_t* g_zob[64];
+
+static int zobrist_table(int dev, const uint64_t** out) {
+  if (!g_zob[dev]) {
+    uint64_t h[16 * 256];
+    uint64_t st = 0x5EED0F2A11CE5EEDULL;
+    for (int i = 0; i < 16 * 256; ++i) {
+      st += 0x9E3779B97F4A7C15ULL;
+      uint64_t z = st;
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+      h[i] = z ^ (z >> 31);
+    }
+    hipError_t e;
+    if ((e = hipMalloc(&g_zob[dev], sizeof(h))) != hipSuccess) return hip_fail(e, "hipMalloc(zobrist)");
+    if ((e = hipMemcpy(g_zob[dev], h, sizeof(h), hipMemcpyHostToDevice)) != hipSuccess)
+      return hip_fail(e, "hipMemcpy(zobrist)");
+  }
+  *out = g_zob[dev];
+  return GO2_OK;
 }
 
 int Workspace::ensure(size_t frame_bytes) {
@@ -849,6 +988,14 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   args.total_lanes = (unsigned int)total_lanes;
   args.limit_ticks = 0;
   args.max_pops = cfg.max_pops_per_trip > 0 ? cfg.max_pops_per_trip : 2;
+  if (cfg.tt_log2_buckets > 0) {
+    if ((rc = ws.ensure_tt(cfg.tt_log2_buckets, stream)) != GO2_OK) return rc;
+    if ((rc = zobrist_table(dev, &args.zob)) != GO2_OK) return rc;
+    args.tt = reinterpret_cast<ulonglong2*>(ws.tt);
+    args.tt_mask = (1ULL << cfg.tt_log2_buckets) - 1;
+    args.tt_gen = ws.tt_gen;
+    args.tt_min = max(max(cfg.tt_min_empties > 0 ? cfg.tt_min_empties : 12, split_min), 4);
+  }
   if (cfg.ordering == GO2_ORDER_LEARNED) {
     const EvalTables& t = g_eval[dev];
     if (!t.w) return fail(GO2_E_STATE, "GO2_ORDER_LEARNED needs go2_set_evaluator() on this device");
@@ -910,6 +1057,10 @@ int collect_stats(Workspace& ws, hipStream_t stream, hipEvent_t ev0, hipEvent_t 
     stats->iterations = h.iterations;
     stats->steals = h.steals;
     stats->lane_slots = h.lane_slots;
+    stats->tt_probes = h.tt_probes;
+    stats->tt_hits = h.tt_hits;
+    stats->tt_cuts = h.tt_cuts;
+    stats->tt_stores = h.tt_stores;
     stats->grid_waves = ws.grid_waves;
     float ms = 0.f;
     if (ev0 && ev1 && hipEventElapsedTime(&ms, ev0, ev1) == hipSuccess) stats->kernel_ms = ms;

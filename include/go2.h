@@ -68,7 +68,10 @@ typedef struct go2_config {
                                    units) + replies*learned_key_mobility; 0 = auto */
   int32_t learned_key_mobility;
   int32_t max_pops_per_trip;    /* finished nodes a lane returns from per loop trip; 0 = auto (2) */
-  int32_t reserved[3];
+  int32_t tt_log2_buckets;      /* transposition table of 2^n 16-byte buckets shared by the whole
+                                   launch (Zobrist keys, lock-free); 0 = no table */
+  int32_t tt_min_empties;       /* probe/store nodes with >= this many empties; 0 = auto (12) */
+  int32_t reserved[1];
 } go2_config;
 
 /* Counters of one solve.  Replaces the reference's `total` node counter
@@ -82,7 +85,11 @@ typedef struct go2_stats {
   uint64_t steals;      /* subtrees handed from a busy lane to an idle one */
   uint64_t lane_slots;  /* lanes launched x loop trips of their wavefront (busy + idle) */
   uint64_t grid_waves;  /* wavefronts launched */
-  uint64_t reserved[5];
+  uint64_t tt_probes;   /* transposition-table lookups */
+  uint64_t tt_hits;     /* lookups that found the position */
+  uint64_t tt_cuts;     /* hits that settled the child without searching it */
+  uint64_t tt_stores;   /* entries written */
+  uint64_t reserved[1];
 } go2_stats;
 
 /* ---- library ---------------------------------------------------------------- */
