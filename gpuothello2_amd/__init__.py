@@ -73,7 +73,8 @@ class Config(ctypes.Structure):
         ("max_pops_per_trip", ctypes.c_int32),
         ("tt_log2_buckets", ctypes.c_int32),
         ("tt_min_empties", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 1),
+        ("graph_launch", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 8),
     ]
 
 

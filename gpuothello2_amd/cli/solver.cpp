@@ -18,9 +18,12 @@
 // Think mode (five arguments, main.cu:180-183) is not part of this solver and
 // is rejected with exit code 2.
 //
-// Search knobs come from the environment (GO2_ORDERING=static|mobility|learned,
-// GO2_BLOCK_SIZE, GO2_MOBILITY_MIN_EMPTIES, GO2_LEARNED_MIN_EMPTIES); the
-// defaults are go2_config_default()'s.
+// Search knobs come from the environment: GO2_CONFIG=file.json (integer
+// go2_config fields, as the autotuner writes them), then GO2_ORDERING=
+// static|mobility|learned, GO2_BLOCK_SIZE, GO2_MOBILITY_MIN_EMPTIES,
+// GO2_LEARNED_MIN_EMPTIES; the defaults are go2_config_default()'s.
+// GO2_ORDERING=learned needs GO2_WEIGHTS=<raw file: 167265 int16 weights,
+// 64 int16 biases, little endian> (scale 64 per disc).
 #include <go2.h>
 
 #include <algorithm>
@@ -80,6 +83,53 @@ std::vector<go2_board> load_boards(const char* path) {
   return v;
 }
 
+// Integer fields of a flat JSON object {"name": value, ...}.
+void apply_json_config(const char* path, go2_config* c) {
+  File f(fopen(path, "r"));
+  if (!f) die(2, std::string("cannot open GO2_CONFIG ") + path);
+  std::string text;
+  char buf[4096];
+  size_t got;
+  while ((got = fread(buf, 1, sizeof buf, f.get())) > 0) text.append(buf, got);
+  struct Field {
+    const char* name;
+    int32_t* slot;
+  } fields[] = {{"ordering", &c->ordering},
+                {"mobility_min_empties", &c->mobility_min_empties},
+                {"learned_min_empties", &c->learned_min_empties},
+                {"block_size", &c->block_size},
+                {"blocks_per_cu", &c->blocks_per_cu},
+                {"positions_per_wave", &c->positions_per_wave},
+                {"split_min_empties", &c->split_min_empties},
+                {"lds_levels", &c->lds_levels},
+                {"time_limit_ms", &c->time_limit_ms},
+                {"split_cut_after", &c->split_cut_after},
+                {"learned_key_eval", &c->learned_key_eval},
+                {"learned_key_mobility", &c->learned_key_mobility},
+                {"max_pops_per_trip", &c->max_pops_per_trip},
+                {"tt_log2_buckets", &c->tt_log2_buckets},
+                {"tt_min_empties", &c->tt_min_empties},
+                {"graph_launch", &c->graph_launch}};
+  size_t pos = 0;
+  while ((pos = text.find('"', pos)) != std::string::npos) {
+    const size_t end = text.find('"', pos + 1);
+    if (end == std::string::npos) break;
+    const std::string key = text.substr(pos + 1, end - pos - 1);
+    const size_t colon = text.find(':', end);
+    if (colon == std::string::npos) break;
+    char* stop = nullptr;
+    const long v = strtol(text.c_str() + colon + 1, &stop, 10);
+    bool known = false;
+    for (auto& fd : fields)
+      if (key == fd.name) {
+        *fd.slot = (int32_t)v;
+        known = true;
+      }
+    if (!known) die(2, "GO2_CONFIG: unknown field " + key);
+    pos = (size_t)(stop - text.c_str());
+  }
+}
+
 int env_int(const char* name, int def) {
   const char* s = getenv(name);
   return s && *s ? atoi(s) : def;
@@ -109,6 +159,8 @@ int main(int argc, char** argv) {
 
   go2_config cfg;
   go2_config_default(&cfg);
+  cfg.time_limit_ms = 0;  // a command-line job runs to completion
+  if (const char* path = getenv("GO2_CONFIG")) apply_json_config(path, &cfg);
   if (const char* o = getenv("GO2_ORDERING")) {
     if (!strcmp(o, "static")) cfg.ordering = GO2_ORDER_STATIC;
     else if (!strcmp(o, "mobility")) cfg.ordering = GO2_ORDER_MOBILITY;
@@ -118,6 +170,18 @@ int main(int argc, char** argv) {
   cfg.block_size = env_int("GO2_BLOCK_SIZE", cfg.block_size);
   cfg.mobility_min_empties = env_int("GO2_MOBILITY_MIN_EMPTIES", cfg.mobility_min_empties);
   cfg.learned_min_empties = env_int("GO2_LEARNED_MIN_EMPTIES", cfg.learned_min_empties);
+  if (cfg.ordering == GO2_ORDER_LEARNED) {
+    const char* wpath = getenv("GO2_WEIGHTS");
+    if (!wpath) die(2, "GO2_ORDERING=learned needs GO2_WEIGHTS");
+    File wf(fopen(wpath, "rb"));
+    if (!wf) die(2, std::string("cannot open GO2_WEIGHTS ") + wpath);
+    int ni = 0, nw = 0, nb = 0;
+    go2_pattern_info(&ni, &nw, &nb);
+    std::vector<int16_t> w((size_t)nw), b((size_t)nb);
+    if (fread(w.data(), 2, w.size(), wf.get()) != w.size() || fread(b.data(), 2, b.size(), wf.get()) != b.size())
+      die(2, "GO2_WEIGHTS is shorter than the evaluator");
+    if (go2_set_evaluator(w.data(), b.data(), 64.0f) != GO2_OK) die(2, go2_last_error());
+  }
 
   const size_t n = boards.size();
   fprintf(stderr, "n = %zu\n", n);

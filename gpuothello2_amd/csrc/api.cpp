@@ -106,6 +106,7 @@ struct go2_task {
   Workspace ws;
   IoBufs bufs;
   Events ev;
+  GraphCache graph;
   bool launched = false;
 };
 
@@ -244,7 +245,7 @@ int go2_task_launch(go2_task* t) {
   int rc = GO2_OK;
   if (t->count)
     rc = launch_solve(t->bufs.boards, t->count, t->max_depth, t->bufs.scores, t->cfg, t->ws,
-                      t->stream, t->ev.a, t->ev.b);
+                      t->stream, t->ev.a, t->ev.b, t->cfg.graph_launch ? &t->graph : nullptr);
   if (rc == GO2_OK) t->launched = true;
   return rc;
 }
@@ -284,10 +285,9 @@ int go2_task_download(go2_task* t, int8_t* scores, go2_stats* stats) {
 
 void go2_task_destroy(go2_task* t) {
   if (!t) return;
-  if (t->stream) {
-    (void)hipStreamSynchronize(t->stream);
-    (void)hipStreamDestroy(t->stream);
-  }
+  if (t->stream) (void)hipStreamSynchronize(t->stream);
+  t->graph.reset();
+  if (t->stream) (void)hipStreamDestroy(t->stream);
   delete t;
 }
 

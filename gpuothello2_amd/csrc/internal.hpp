@@ -24,6 +24,8 @@ struct SolveCounters {
   unsigned long long tt_hits;
   unsigned long long tt_cuts;
   unsigned long long tt_stores;
+  unsigned int tt_gen;  // launch generation 1..255 (advanced by the reset kernel, never zeroed)
+  unsigned int pad;
 };
 
 constexpr unsigned int kTimedOut = 0xFFFFFFFFu;
@@ -55,11 +57,26 @@ struct Workspace {
   int ensure_tt(int log2_buckets, hipStream_t stream);
 };
 
-// Enqueue counter reset + solve kernel on `stream`; events bracket the solve
-// kernel when non-null.
+// A launch sequence (counter reset + solve kernel) captured in a HIP graph,
+// re-captured when any launch argument changes.
+struct GraphCache {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  unsigned char key[512] = {0};
+  size_t key_len = 0;
+  GraphCache() = default;
+  GraphCache(const GraphCache&) = delete;
+  GraphCache& operator=(const GraphCache&) = delete;
+  ~GraphCache();
+  void reset();
+};
+
+// Enqueue counter reset + solve kernel on `stream` (directly, or as a replay
+// of a captured graph when `graph` is non-null); events bracket the launch
+// when non-null.
 int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_scores,
                  const go2_config& cfg, Workspace& ws, hipStream_t stream, hipEvent_t ev0,
-                 hipEvent_t ev1);
+                 hipEvent_t ev1, GraphCache* graph = nullptr);
 
 // Copy counters back (stream-ordered, then synchronises the stream) and fill stats.
 int collect_stats(Workspace& ws, hipStream_t stream, hipEvent_t ev0, hipEvent_t ev1,

@@ -42,6 +42,8 @@
 //     are folded into the child frame (board swapped, window kept, value not
 //     negated), the recurrence of solver.cu:65-73,162-165,214-221,246-256.
 //   * A wall-clock deadline (s_memrealtime) bounds every launch.
+#include <cstring>
+
 #include "bitboard.hpp"
 #include "internal.hpp"
 #include "pattern.hpp"
@@ -144,7 +146,7 @@ struct SolveArgs {
   ulonglong2* tt;
   unsigned long long tt_mask;
   int tt_min;
-  unsigned int tt_gen;
+  const unsigned int* tt_gen;  // launch generation (device, set by the reset kernel)
   const uint64_t* zob;  // Zobrist keys per (byte position 0..15, byte value)
   SolveCounters* ctr;
   // HBM frames of levels >= lds_levels+3: [level - lds_levels - 3][total_lanes]
@@ -335,6 +337,7 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
     if (E >= a.split_min) set_sx(E, sx_init(alpha));
   };
 
+  const unsigned int tt_gen = a.tt ? *a.tt_gen : 0u;
   unsigned int trip = 0;
   const unsigned long long deadline = __builtin_amdgcn_s_memrealtime() + a.limit_ticks;
   for (;;) {
@@ -616,7 +619,7 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
         ++c_probes;
         const uint64_t h = zobrist(cP, cO, a.zob);
         const ulonglong2 b = a.tt[h & a.tt_mask];
-        if ((b.x ^ b.y) == h && (uint32_t)((b.y >> 24) & 255) == a.tt_gen) {
+        if ((b.x ^ b.y) == h && (uint32_t)((b.y >> 24) & 255) == tt_gen) {
           ++c_hits;
           const int lo = (int)(b.y & 255) - 64, up = (int)((b.y >> 8) & 255) - 64;
           int tv = 0;
@@ -712,7 +715,7 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
           ++c_stores;
           const int lo = best > alpha0 ? best : -64, up = best < beta ? best : 64;
           const uint64_t d = (uint64_t)(lo + 64) | ((uint64_t)(up + 64) << 8) | ((uint64_t)bestmv << 16) |
-                             ((uint64_t)a.tt_gen << 24) | ((uint64_t)E << 32);
+                             ((uint64_t)tt_gen << 24) | ((uint64_t)E << 32);
           const uint64_t h = zobrist(P, O, a.zob);
           a.tt[h & a.tt_mask] = make_ulonglong2(h ^ d, d);
         }
@@ -784,6 +787,7 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
 }
 
 __global__ void reset_counters_kernel(SolveCounters* c) {
+  c->tt_gen = c->tt_gen % 255u + 1u;
   c->next = 0;
   c->error = 0;
   c->positions = 0;
@@ -878,9 +882,11 @@ int Workspace::ensure_tt(int log2_buckets, hipStream_t stream) {
     tt_log2 = log2_buckets;
     tt_gen = 0;
   }
-  // generations 1..255; a wrap clears the table (a fill kernel, no memset node)
-  if (++tt_gen > 255 || tt_gen == 1) {
-    tt_gen = 1;
+  // The reset kernel of the coming launch moves the device generation to
+  // tt_gen % 255 + 1; the host mirrors it and clears the table (a fill
+  // kernel, outside any captured graph) before generation 1 comes round.
+  tt_gen = tt_gen % 255u + 1u;
+  if (tt_gen == 1) {
     const unsigned long long n = 1ULL << tt_log2;
     hipLaunchKernelGGL(clear_tt_kernel, dim3(4096), dim3(256), 0, stream, (ulonglong2*)tt, n);
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "clear table launch");
@@ -913,8 +919,10 @@ static int zobrist_table(int dev, const uint64_t** out) {
 
 int Workspace::ensure(size_t frame_bytes) {
   hipError_t e;
-  if (!ctr && (e = hipMalloc(&ctr, sizeof(SolveCounters))) != hipSuccess)
-    return hip_fail(e, "hipMalloc(counters)");
+  if (!ctr) {
+    if ((e = hipMalloc(&ctr, sizeof(SolveCounters))) != hipSuccess) return hip_fail(e, "hipMalloc(counters)");
+    if ((e = hipMemset(ctr, 0, sizeof(SolveCounters))) != hipSuccess) return hip_fail(e, "hipMemset(counters)");
+  }
   if (frame_bytes > frames_bytes) {
     if (frames) (void)hipFree(frames);
     frames = nullptr;
@@ -927,7 +935,7 @@ int Workspace::ensure(size_t frame_bytes) {
 
 int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_scores,
                  const go2_config& cfg, Workspace& ws, hipStream_t stream, hipEvent_t ev0,
-                 hipEvent_t ev1) {
+                 hipEvent_t ev1, GraphCache* graph) {
   if (n > 0xFFFFFF00ULL) return fail(GO2_E_ARG, "too many boards in one solve (max 2^32-256)");
   KernelFn fn = nullptr;
   const int maxe = stack_levels_for(max_depth, &fn, cfg.ordering);
@@ -993,7 +1001,7 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
     if ((rc = zobrist_table(dev, &args.zob)) != GO2_OK) return rc;
     args.tt = reinterpret_cast<ulonglong2*>(ws.tt);
     args.tt_mask = (1ULL << cfg.tt_log2_buckets) - 1;
-    args.tt_gen = ws.tt_gen;
+    args.tt_gen = &ws.ctr->tt_gen;
     args.tt_min = max(max(cfg.tt_min_empties > 0 ? cfg.tt_min_empties : 12, split_min), 4);
   }
   if (cfg.ordering == GO2_ORDER_LEARNED) {
@@ -1023,20 +1031,70 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   if (off > ws.frames_bytes) {
     rc = ws.ensure(off);
     if (rc != GO2_OK) return rc;
-    return launch_solve(d_boards, n, max_depth, d_scores, cfg, ws, stream, ev0, ev1);
+    return launch_solve(d_boards, n, max_depth, d_scores, cfg, ws, stream, ev0, ev1, graph);
   }
   // s_memrealtime runs at 100 MHz; each wavefront adds its own start time.
   if (cfg.time_limit_ms > 0) args.limit_ticks = (unsigned long long)cfg.time_limit_ms * 100000ULL;
   ws.grid_waves = waves;
 
-  reset_counters_kernel<<<1, 1, 0, stream>>>(ws.ctr);
-  if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "reset_counters launch");
+  auto enqueue = [&]() -> int {
+    reset_counters_kernel<<<1, 1, 0, stream>>>(ws.ctr);
+    hipError_t le = hipGetLastError();
+    if (le != hipSuccess) return hip_fail(le, "reset_counters launch");
+    hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(bs), lds, stream, args);
+    if ((le = hipGetLastError()) != hipSuccess) return hip_fail(le, "solve kernel launch");
+    return GO2_OK;
+  };
+  if (!graph) {
+    if (ev0 && (e = hipEventRecord(ev0, stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+    if ((rc = enqueue()) != GO2_OK) return rc;
+    if (ev1 && (e = hipEventRecord(ev1, stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+    return GO2_OK;
+  }
+  // Replay a captured graph of the same launch; capture it when the
+  // kernel, shape or any argument changed.
+  unsigned char key[512] = {0};
+  size_t klen = 0;
+  auto put = [&](const void* p, size_t b) {
+    memcpy(key + klen, p, b);
+    klen += b;
+  };
+  const void* fnp = (const void*)fn;
+  put(&fnp, sizeof(fnp));
+  put(&grid, sizeof(grid));
+  put(&bs, sizeof(bs));
+  put(&lds, sizeof(lds));
+  put(&args, sizeof(args));
+  if (!graph->exec || graph->key_len != klen || memcmp(graph->key, key, klen) != 0) {
+    graph->reset();
+    if ((e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+      return hip_fail(e, "hipStreamBeginCapture");
+    rc = enqueue();
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(stream, &g);
+    if (rc != GO2_OK) return rc;
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    graph->graph = g;
+    if ((e = hipGraphInstantiate(&graph->exec, g, nullptr, nullptr, 0)) != hipSuccess)
+      return hip_fail(e, "hipGraphInstantiate");
+    memcpy(graph->key, key, klen);
+    graph->key_len = klen;
+  }
   if (ev0 && (e = hipEventRecord(ev0, stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
-  hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(bs), lds, stream, args);
-  if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "solve kernel launch");
+  if ((e = hipGraphLaunch(graph->exec, stream)) != hipSuccess) return hip_fail(e, "hipGraphLaunch");
   if (ev1 && (e = hipEventRecord(ev1, stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
   return GO2_OK;
 }
+
+void GraphCache::reset() {
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  exec = nullptr;
+  graph = nullptr;
+  key_len = 0;
+}
+
+GraphCache::~GraphCache() { reset(); }
 
 int collect_stats(Workspace& ws, hipStream_t stream, hipEvent_t ev0, hipEvent_t ev1,
                   go2_stats* stats) {

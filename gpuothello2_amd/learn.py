@@ -153,6 +153,14 @@ def load(path: str = WEIGHTS_PATH):
     return d["w16"], d["b16"], float(d["scale"]), json.loads(str(d["meta"]))
 
 
+def export_raw(out_path: str, path: str = WEIGHTS_PATH) -> None:
+    """Raw little-endian int16 weights then biases (the CLI's GO2_WEIGHTS file)."""
+    w16, b16, _, _ = load(path)
+    with open(out_path, "wb") as f:
+        f.write(np.ascontiguousarray(w16, dtype="<i2").tobytes())
+        f.write(np.ascontiguousarray(b16, dtype="<i2").tobytes())
+
+
 def install(path: str = WEIGHTS_PATH) -> dict:
     """Install the saved int16 evaluator for ORDER_LEARNED solves; returns its metadata."""
     w16, b16, scale, meta = load(path)

@@ -71,7 +71,9 @@ typedef struct go2_config {
   int32_t tt_log2_buckets;      /* transposition table of 2^n 16-byte buckets shared by the whole
                                    launch (Zobrist keys, lock-free); 0 = no table */
   int32_t tt_min_empties;       /* probe/store nodes with >= this many empties; 0 = auto (12) */
-  int32_t reserved[1];
+  int32_t graph_launch;         /* tasks capture their launch (counter reset + solve kernel) in a
+                                   HIP graph once and replay it on later launches */
+  int32_t reserved[8];
 } go2_config;
 
 /* Counters of one solve.  Replaces the reference's `total` node counter
