@@ -74,7 +74,8 @@ class Config(ctypes.Structure):
         ("tt_log2_buckets", ctypes.c_int32),
         ("tt_min_empties", ctypes.c_int32),
         ("graph_launch", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 8),
+        ("export_min_empties", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 7),
     ]
 
 
@@ -92,7 +93,7 @@ class Stats(ctypes.Structure):
         ("tt_hits", ctypes.c_uint64),
         ("tt_cuts", ctypes.c_uint64),
         ("tt_stores", ctypes.c_uint64),
-        ("reserved", ctypes.c_uint64 * 1),
+        ("exports", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:

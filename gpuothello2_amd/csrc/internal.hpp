@@ -25,7 +25,8 @@ struct SolveCounters {
   unsigned long long tt_cuts;
   unsigned long long tt_stores;
   unsigned int tt_gen;  // launch generation 1..255 (advanced by the reset kernel, never zeroed)
-  unsigned int pad;
+  unsigned int active_waves;  // wavefronts not idle (cross-wavefront help)
+  unsigned long long exports;
 };
 
 constexpr unsigned int kTimedOut = 0xFFFFFFFFu;
@@ -49,12 +50,16 @@ struct Workspace {
   void* tt = nullptr;
   int tt_log2 = 0;
   unsigned int tt_gen = 0;
+  // cross-wavefront help: slots and mailboxes
+  void* help = nullptr;
+  size_t help_bytes = 0;
   Workspace() = default;
   Workspace(const Workspace&) = delete;
   Workspace& operator=(const Workspace&) = delete;
   ~Workspace();
   int ensure(size_t frame_bytes);
   int ensure_tt(int log2_buckets, hipStream_t stream);
+  int ensure_help(size_t bytes, hipStream_t stream);
 };
 
 // A launch sequence (counter reset + solve kernel) captured in a HIP graph,

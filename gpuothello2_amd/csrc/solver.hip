@@ -148,6 +148,18 @@ struct SolveArgs {
   int tt_min;
   const unsigned int* tt_gen;  // launch generation (device, set by the reset kernel)
   const uint64_t* zob;  // Zobrist keys per (byte position 0..15, byte value)
+  // cross-wavefront help (null = off): once the board queue is empty, busy
+  // wavefronts export siblings of nodes with >= ext_min empties to `slots`;
+  // idle wavefronts claim them and report through per-frame mailboxes
+  // [E - ext_min][total_lanes]
+  struct HelpSlot* slots;
+  unsigned int nslots;
+  int ext_min;
+  int* mb_best;
+  unsigned int* mb_done;
+  unsigned int* mb_cut;
+  unsigned int* mb_slot;  // slot index << 6 | move square of the frame's exported sibling
+  unsigned int grid_waves;
   SolveCounters* ctr;
   // HBM frames of levels >= lds_levels+3: [level - lds_levels - 3][total_lanes]
   uint64_t* g_fb;
@@ -207,6 +219,36 @@ template <class T>
 __device__ __forceinline__ void hbm_st(T* p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
 }
+
+// One exported sibling: board, window, expected type and the owner's frame.
+struct HelpSlot {
+  uint64_t P, O;
+  int alpha, beta, ctype, owner_E;
+  unsigned int owner;  // global lane id of the owner
+  unsigned int state;  // kSlotEmpty / kSlotWriting / full (tag << 2 | kSlotFull) / kSlotTaken
+  unsigned int pad[8];
+};
+constexpr unsigned int kSlotEmpty = 0, kSlotWriting = 1, kSlotFull = 2, kSlotTaken = 3;
+// A full slot carries its owner's tag (global lane id and frame level), so
+// an owner taking back an unclaimed sibling can never cancel another
+// owner's item that reused the slot.
+__device__ __forceinline__ unsigned int slot_full(unsigned int g, int e) {
+  return ((g << 5 | (unsigned int)e) << 2) | kSlotFull;
+}
+
+template <class T>
+__device__ __forceinline__ T ld_agent(T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ void st_agent(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Split word bit 29: a sibling of this node is out with another wavefront
+// (at most one per node; its owner takes it back if nobody claims it).
+__device__ __forceinline__ int sx_ext(uint32_t x) { return (int)((x >> 29) & 1u); }
+constexpr uint32_t kSxExt = 1u << 29;
 
 // Board of the parent of (P, O), from the frame word and flip set the parent
 // stored when it moved to (P, O).
@@ -306,6 +348,11 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
   bool pop_pending = false;     // more finished nodes to return from next trip
   int ndone = 0, ntype = kPV;   // finished children (saturating at 3) and expected type
   int owner = -1, owner_E = 0;  // owner lane and frame level of a helper's task (-1: own board)
+  bool owner_ext = false;       // the owner is in another wavefront (owner = its global lane id)
+  uint32_t ext_mask = 0;        // bit E-3: frame E has siblings out with other wavefronts
+  bool wave_idle = false;       // wavefront-uniform: counted out of active_waves
+  bool demand = false;          // wavefront-uniform: some wavefront is idle
+  unsigned int poll_base = gtid - (unsigned int)lane;  // wavefront-uniform
   int alpha0 = 0;               // current node's alpha at entry
   int bestmv = 63;              // move that gave the current node's best value (63 = none)
   unsigned int idx = 0;
@@ -317,7 +364,8 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
   bool report_aborted = false;
   bool queue_done = false;      // wavefront-uniform
   unsigned long long c_pos = 0, c_nodes = 0, c_steps = 0, c_iters = 0, c_steals = 0, c_slots = 0;
-  unsigned long long c_probes = 0, c_hits = 0, c_cuts = 0, c_stores = 0;
+  unsigned long long c_probes = 0, c_hits = 0, c_cuts = 0, c_stores = 0, c_exports = 0;
+  auto mb_idx = [&](int e, unsigned int g) { return (size_t)(e - a.ext_min) * a.total_lanes + g; };
 
   // A node at level E with legal moves `cm` becomes the current node.
   auto enter_node = [&](uint64_t cm, int type) {
@@ -335,6 +383,50 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
     alpha0 = alpha;
     bestmv = 63;
     if (E >= a.split_min) set_sx(E, sx_init(alpha));
+    if (a.slots && E >= a.ext_min) {
+      const size_t m = mb_idx(E, gtid);
+      st_agent(&a.mb_best[m], kNoValue);
+      st_agent(&a.mb_done[m], 0u);
+      st_agent(&a.mb_cut[m], 0u);
+    }
+  };
+
+  // This lane starts searching child (cP, cO) of frame oe of its owner, with
+  // the owner's window (ca, cb) on that frame.
+  auto start_child_task = [&](uint64_t cP, uint64_t cO, int ca, int cb, int ct, int own, int oe,
+                              bool ext) {
+    ++c_nodes;
+    E = oe - 1;
+    Eroot = E;
+    owner = own;
+    owner_ext = ext;
+    owner_E = oe;
+    dead_level = 64;
+    const uint64_t cm = mobility(cP, cO);
+    if (cm) {
+      P = cP;
+      O = cO;
+      alpha = -cb;
+      beta = -ca;
+      negate = true;
+      enter_node(cm, ct);
+      mode = SEARCH;
+    } else {
+      const uint64_t om = mobility(cO, cP);
+      if (om) {
+        P = cO;
+        O = cP;
+        alpha = ca;
+        beta = cb;
+        negate = false;
+        enter_node(om, ct);
+        mode = SEARCH;
+      } else {
+        report_value = -final_score(cP, cO);
+        report_aborted = false;
+        mode = REPORT;
+      }
+    }
   };
 
   const unsigned int tt_gen = a.tt ? *a.tt_gen : 0u;
@@ -342,7 +434,8 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
   const unsigned long long deadline = __builtin_amdgcn_s_memrealtime() + a.limit_ticks;
   for (;;) {
     // ---- time limit (checked every 256 trips; uniform per wavefront) ----
-    if (a.limit_ticks && (++trip & 255) == 0) {
+    ++trip;
+    if (a.limit_ticks && (trip & 255) == 0) {
       if (__builtin_amdgcn_s_memrealtime() > deadline) {
         if (lane == 0) atomicExch(&a.ctr->error, kTimedOut);
         break;
@@ -363,6 +456,7 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
           const unsigned int i = base + (unsigned int)__builtin_popcountll(want & __lanemask_lt());
           if (i < a.n) {
             idx = i;
+            owner_ext = false;
             const go2_board b = a.boards[i];
             P = b.player;
             O = b.opponent;
@@ -404,11 +498,102 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
         }
       }
     } else if (__ballot(mode != IDLE) == 0) {
-      break;
+      if (!a.slots) break;
+      // Whole wavefront idle: claim siblings other wavefronts exported, or
+      // leave once no wavefront is active (an active one may still export;
+      // an exported sibling always has its active owner waiting for it).
+      if (!wave_idle) {
+        wave_idle = true;
+        if (lane == 0) atomicSub(&a.ctr->active_waves, 1u);
+      }
+      const unsigned int sidx = (poll_base + (unsigned int)lane) % a.nslots;
+      poll_base += 64;
+      bool won = false;
+      const unsigned int sst = ld_agent(&a.slots[sidx].state);
+      if ((sst & 3u) == kSlotFull) {
+        unsigned int expect = sst;
+        won = __hip_atomic_compare_exchange_strong(&a.slots[sidx].state, &expect, kSlotTaken, __ATOMIC_ACQUIRE,
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      if (__ballot(won)) {
+        if (lane == 0) atomicAdd(&a.ctr->active_waves, 1u);
+        wave_idle = false;
+        if (won) {
+          HelpSlot* hs = &a.slots[sidx];
+          const uint64_t cP = ld_agent(&hs->P), cO = ld_agent(&hs->O);
+          const int ca = ld_agent(&hs->alpha), cb = ld_agent(&hs->beta);
+          const int ct = ld_agent(&hs->ctype), oe = ld_agent(&hs->owner_E);
+          const unsigned int og = ld_agent(&hs->owner);
+          __hip_atomic_store(&hs->state, kSlotEmpty, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+          ++c_steals;
+          start_child_task(cP, cO, ca, cb, ct, (int)og, oe, true);
+        }
+      } else {
+        unsigned int act = 0;
+        if (lane == 0) act = ld_agent(&a.ctr->active_waves);
+        act = (unsigned int)__builtin_amdgcn_readlane((int)act, 0);
+        if (act == 0) break;
+        __builtin_amdgcn_s_sleep(16);
+      }
     }
 
     // ---- 2. helpers report to their owners, one at a time ----
-    uint64_t rm = __ballot(mode == REPORT);
+    if (mode == REPORT && owner_ext) {
+      const size_t m = mb_idx(owner_E, (unsigned int)owner);
+      if (!report_aborted) atomicMax(&a.mb_best[m], report_value);
+      __hip_atomic_fetch_add(&a.mb_done[m], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      mode = IDLE;
+    }
+    if (a.slots && (trip & 15) == 0) {
+      // Owners collect siblings that came back from other wavefronts; helpers
+      // of other wavefronts stop when their owner's node was cut.
+      if (ext_mask) {
+        uint32_t em = ext_mask;
+        while (em) {
+          const int e = __builtin_ctz(em) + 3;
+          em &= em - 1;
+          const size_t m = mb_idx(e, gtid);
+          const unsigned int got =
+              __hip_atomic_exchange(&a.mb_done[m], 0u, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+          if (got) {
+            uint32_t sx = (get_sx(e) - 1u) & ~kSxExt;
+            const int mb = ld_agent(&a.mb_best[m]);
+            if (mb >= -64) sx = sx_with_best(sx, mb);
+            set_sx(e, sx);
+            --helpers_out;
+            ext_mask &= ~(1u << (e - 3));
+          } else if ((mode == WAIT && E == e) || mode == ABORT) {
+            // Nobody claimed it while this lane waits: take it back.
+            const unsigned int sl = ld_agent(&a.mb_slot[m]);
+            unsigned int expect = slot_full(gtid, e);
+            if (__hip_atomic_compare_exchange_strong(&a.slots[sl >> 6].state, &expect, kSlotEmpty,
+                                                     __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT)) {
+              set_sx(e, (get_sx(e) - 1u) & ~kSxExt);
+              --helpers_out;
+              ext_mask &= ~(1u << (e - 3));
+              if (mode == WAIT && alpha < beta) {
+                moves |= 1ULL << (sl & 63);
+                mode = SEARCH;
+              }
+            }
+          }
+        }
+      }
+      if (owner_ext && (mode == SEARCH || mode == WAIT) &&
+          ld_agent(&a.mb_cut[mb_idx(owner_E, (unsigned int)owner)])) {
+        mode = ABORT;
+        dead_level = 0;
+        dead_new = true;
+        split_mask = 0;
+      }
+      if (lane == 0) {
+        const unsigned int act = ld_agent(&a.ctr->active_waves);
+        demand = act < a.grid_waves;
+      }
+      demand = __builtin_amdgcn_readlane((int)demand, 0);
+    }
+    uint64_t rm = __ballot(mode == REPORT && !owner_ext);
     while (rm) {
       const int r = __builtin_ctzll(rm);
       rm &= rm - 1;
@@ -426,13 +611,21 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
     }
 
     // ---- 3. cut frames stop their helpers (cascades one level per trip) ----
+    if (dead_new && ext_mask) {
+      uint32_t em = ext_mask;
+      while (em) {
+        const int e = __builtin_ctz(em) + 3;
+        em &= em - 1;
+        if (dead_level == 0 || dead_level == e) st_agent(&a.mb_cut[mb_idx(e, gtid)], 1u);
+      }
+    }
     uint64_t dm = __ballot(dead_new);
     while (dm) {
       const int l = __builtin_ctzll(dm);
       dm &= dm - 1;
       const int dl = __builtin_amdgcn_readlane(dead_level, l);
       if (lane == l) dead_new = false;
-      if ((mode == SEARCH || mode == WAIT) && owner == l && (dl == 0 || dl == owner_E)) {
+      if ((mode == SEARCH || mode == WAIT) && !owner_ext && owner == l && (dl == 0 || dl == owner_E)) {
         mode = ABORT;
         dead_level = 0;
         dead_new = true;
@@ -495,36 +688,66 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
         ct = __builtin_amdgcn_readlane(ct, victim);
         if (lane == thief) {
           ++c_steals;
-          ++c_nodes;
-          E = e - 1;
-          Eroot = E;
-          owner = victim;
-          owner_E = e;
-          dead_level = 64;
-          const uint64_t cm = mobility(cP, cO);
-          if (cm) {
-            P = cP;
-            O = cO;
-            alpha = -cb;
-            beta = -ca;
-            negate = true;
-            enter_node(cm, ct);
-            mode = SEARCH;
-          } else {
-            const uint64_t om = mobility(cO, cP);
-            if (om) {
-              P = cO;
-              O = cP;
-              alpha = ca;
-              beta = cb;
-              negate = false;
-              enter_node(om, ct);
-              mode = SEARCH;
+          start_child_task(cP, cO, ca, cb, ct, victim, e, false);
+        }
+      }
+    } else if (a.slots && demand && !idle && __ballot((split_mask >> (a.ext_min - 3)) != 0)) {
+      // No idle lane here but an idle wavefront elsewhere: export the
+      // highest offered sibling (>= ext_min empties) to a help slot.
+      uint32_t lv = split_mask >> (a.ext_min - 3);
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) lv |= (uint32_t)__shfl_xor((int)lv, o);
+      lv = (uint32_t)__builtin_amdgcn_readfirstlane((int)lv);
+      const int e = 31 - __builtin_clz(lv) + a.ext_min;
+      const int victim = __builtin_ctzll(__ballot((split_mask >> (e - 3)) & 1));
+      if (lane == victim) {
+        uint64_t mv = get_mv(e), od = get_od(e);
+        const uint32_t w = get_wd(e);
+        const uint32_t sx = get_sx(e);
+        const int fa = frame_val(w, 0), fbeta = frame_val(w, 7);
+        const int am = max(fa, sx_best(sx));
+        if (mv == 0 || am >= fbeta) {
+          split_mask &= ~(1u << (e - 3));
+        } else if (sx_ext(sx) == 0) {
+          // claim an empty slot first
+          unsigned int sidx = (gtid * 2654435761u + trip) % a.nslots;
+          bool got = false;
+          for (int k = 0; k < 8 && !got; ++k) {
+            unsigned int expect = kSlotEmpty;
+            got = __hip_atomic_compare_exchange_strong(&a.slots[sidx].state, &expect, kSlotWriting,
+                                                       __ATOMIC_ACQUIRE, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
+            if (!got) sidx = (sidx + 1) % a.nslots;
+          }
+          if (got) {
+            uint64_t bp = P, bo = O;
+            for (int l = E + 1; l <= e; ++l) undo_move(bp, bo, get_fb(l), get_wd(l));
+            int sq;
+            if (od) {
+              sq = (int)(od & 127) - 1;
+              od >>= 7;
             } else {
-              report_value = -final_score(cP, cO);
-              report_aborted = false;
-              mode = REPORT;
+              sq = pick_static<ORDER>(mv, ~(bp | bo));
             }
+            const uint64_t bit = 1ULL << sq;
+            mv ^= bit;
+            set_mv_od(e, mv, od);
+            set_sx(e, (sx + 1u) | kSxExt);
+            st_agent(&a.mb_slot[mb_idx(e, gtid)], (sidx << 6) | (unsigned int)sq);
+            ++helpers_out;
+            ext_mask |= 1u << (e - 3);
+            if (!mv) split_mask &= ~(1u << (e - 3));
+            const uint64_t f = flip(bp, bo, sq);
+            HelpSlot* hs = &a.slots[sidx];
+            st_agent(&hs->P, bo ^ f);
+            st_agent(&hs->O, bp ^ f ^ bit);
+            st_agent(&hs->alpha, am);
+            st_agent(&hs->beta, fbeta);
+            st_agent(&hs->ctype, child_type((int)(w >> 30), 1));
+            st_agent(&hs->owner_E, e);
+            st_agent(&hs->owner, gtid);
+            __hip_atomic_store(&hs->state, slot_full(gtid, e), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            ++c_exports;
           }
         }
       }
@@ -778,6 +1001,7 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
   atomicAdd(&a.ctr->iterations, c_iters);
   atomicAdd(&a.ctr->steals, c_steals);
   atomicAdd(&a.ctr->lane_slots, c_slots);
+  if (a.slots) atomicAdd(&a.ctr->exports, c_exports);
   if (a.tt) {
     atomicAdd(&a.ctr->tt_probes, c_probes);
     atomicAdd(&a.ctr->tt_hits, c_hits);
@@ -786,7 +1010,9 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
   }
 }
 
-__global__ void reset_counters_kernel(SolveCounters* c) {
+__global__ void reset_counters_kernel(SolveCounters* c, unsigned int waves) {
+  c->active_waves = waves;
+  c->exports = 0;
   c->tt_gen = c->tt_gen % 255u + 1u;
   c->next = 0;
   c->error = 0;
@@ -863,6 +1089,7 @@ Workspace::~Workspace() {
   if (ctr) (void)hipFree(ctr);
   if (frames) (void)hipFree(frames);
   if (tt) (void)hipFree(tt);
+  if (help) (void)hipFree(help);
 }
 
 __global__ void clear_tt_kernel(ulonglong2* tt, unsigned long long n) {
@@ -917,6 +1144,24 @@ static int zobrist_table(int dev, const uint64_t** out) {
   return GO2_OK;
 }
 
+int Workspace::ensure_help(size_t bytes, hipStream_t stream) {
+  if (bytes > help_bytes) {
+    if (help) (void)hipFree(help);
+    help = nullptr;
+    help_bytes = 0;
+    hipError_t e = hipMalloc(&help, bytes);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(help slots)");
+    help_bytes = bytes;
+  }
+  (void)stream;
+  return GO2_OK;
+}
+
+__global__ void clear_slots_kernel(HelpSlot* s, unsigned int n) {
+  for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    s[i].state = kSlotEmpty;
+}
+
 int Workspace::ensure(size_t frame_bytes) {
   hipError_t e;
   if (!ctr) {
@@ -963,9 +1208,12 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   if (cfg.blocks_per_cu > 0 && cfg.blocks_per_cu < per_cu) per_cu = cfg.blocks_per_cu;
   const size_t waves_per_block = (size_t)bs / 64;
   size_t grid = (size_t)cus * per_cu;
-  // Never launch more wavefronts than boards.
+  // Without cross-wavefront help, never launch more wavefronts than boards;
+  // with it, the whole resident grid runs and idle wavefronts take siblings.
+  const bool help = cfg.export_min_empties <= 60 &&
+                    (cfg.export_min_empties > 0 ? max(cfg.export_min_empties, 5) : 12) <= maxe;
   const size_t max_grid = (n + waves_per_block - 1) / waves_per_block;
-  if (grid > max_grid) grid = max_grid;
+  if (!help && grid > max_grid) grid = max_grid;
   if (grid < 1) grid = 1;
   const size_t waves = grid * waves_per_block;
   int K = cfg.positions_per_wave;
@@ -996,6 +1244,29 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   args.total_lanes = (unsigned int)total_lanes;
   args.limit_ticks = 0;
   args.max_pops = cfg.max_pops_per_trip > 0 ? cfg.max_pops_per_trip : 2;
+  const int ext_min = cfg.export_min_empties > 0 ? max(cfg.export_min_empties, split_min) : max(12, split_min);
+  if (ext_min <= maxe && cfg.export_min_empties <= 60) {
+    const unsigned int nslots = 4096;
+    const size_t mlev = (size_t)(maxe + 1 - ext_min);
+    const size_t mb = mlev * total_lanes * 4;
+    const size_t need = (size_t)nslots * sizeof(HelpSlot) + 4 * ((mb + 255) & ~(size_t)255);
+    if ((rc = ws.ensure_help(need, stream)) != GO2_OK) return rc;
+    uint8_t* hb = reinterpret_cast<uint8_t*>(ws.help);
+    args.slots = reinterpret_cast<HelpSlot*>(hb);
+    args.nslots = nslots;
+    args.ext_min = ext_min;
+    size_t o = (size_t)nslots * sizeof(HelpSlot);
+    args.mb_best = reinterpret_cast<int*>(hb + o);
+    o += (mb + 255) & ~(size_t)255;
+    args.mb_done = reinterpret_cast<unsigned int*>(hb + o);
+    o += (mb + 255) & ~(size_t)255;
+    args.mb_cut = reinterpret_cast<unsigned int*>(hb + o);
+    o += (mb + 255) & ~(size_t)255;
+    args.mb_slot = reinterpret_cast<unsigned int*>(hb + o);
+  } else {
+    args.ext_min = 3;  // keeps the (disabled) export test's shift in range
+  }
+  args.grid_waves = (unsigned int)waves;
   if (cfg.tt_log2_buckets > 0) {
     if ((rc = ws.ensure_tt(cfg.tt_log2_buckets, stream)) != GO2_OK) return rc;
     if ((rc = zobrist_table(dev, &args.zob)) != GO2_OK) return rc;
@@ -1038,9 +1309,14 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   ws.grid_waves = waves;
 
   auto enqueue = [&]() -> int {
-    reset_counters_kernel<<<1, 1, 0, stream>>>(ws.ctr);
+    reset_counters_kernel<<<1, 1, 0, stream>>>(ws.ctr, (unsigned int)waves);
     hipError_t le = hipGetLastError();
     if (le != hipSuccess) return hip_fail(le, "reset_counters launch");
+    if (args.slots) {
+      // a launch abandoned at its time limit may leave slots behind
+      clear_slots_kernel<<<16, 256, 0, stream>>>(args.slots, args.nslots);
+      if ((le = hipGetLastError()) != hipSuccess) return hip_fail(le, "clear slots launch");
+    }
     hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(bs), lds, stream, args);
     if ((le = hipGetLastError()) != hipSuccess) return hip_fail(le, "solve kernel launch");
     return GO2_OK;
@@ -1119,6 +1395,7 @@ int collect_stats(Workspace& ws, hipStream_t stream, hipEvent_t ev0, hipEvent_t 
     stats->tt_hits = h.tt_hits;
     stats->tt_cuts = h.tt_cuts;
     stats->tt_stores = h.tt_stores;
+    stats->exports = h.exports;
     stats->grid_waves = ws.grid_waves;
     float ms = 0.f;
     if (ev0 && ev1 && hipEventElapsedTime(&ms, ev0, ev1) == hipSuccess) stats->kernel_ms = ms;

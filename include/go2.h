@@ -73,7 +73,10 @@ typedef struct go2_config {
   int32_t tt_min_empties;       /* probe/store nodes with >= this many empties; 0 = auto (12) */
   int32_t graph_launch;         /* tasks capture their launch (counter reset + solve kernel) in a
                                    HIP graph once and replay it on later launches */
-  int32_t reserved[8];
+  int32_t export_min_empties;   /* once the board queue is empty, busy wavefronts hand siblings of
+                                   nodes with >= this many empties to idle wavefronts; 0 = auto
+                                   (12), > 60 = never */
+  int32_t reserved[7];
 } go2_config;
 
 /* Counters of one solve.  Replaces the reference's `total` node counter
@@ -91,7 +94,7 @@ typedef struct go2_stats {
   uint64_t tt_hits;     /* lookups that found the position */
   uint64_t tt_cuts;     /* hits that settled the child without searching it */
   uint64_t tt_stores;   /* entries written */
-  uint64_t reserved[1];
+  uint64_t exports;     /* siblings handed to other wavefronts */
 } go2_stats;
 
 /* ---- library ---------------------------------------------------------------- */

@@ -197,3 +197,23 @@ def test_helpers_on_a_single_board():
                             cfg("mobility", positions_per_wave=1, split_min_empties=6))
         assert got[0] == want[i]
         assert st["steals"] > 0
+
+
+# Cross-wavefront help: few busy wavefronts (one lane per board) and many
+# idle ones that take exported siblings; scores must stay exact.
+@pytest.mark.parametrize("ppw,emin", [(64, 10), (16, 11), (0, 12), (4, 10)])
+def test_cross_wavefront_help_keeps_scores_exact(ppw, emin):
+    rows = [ol.generate(808, 3, 1, int(ol.empties_for(808, 3, i, 10, 14)), start=i)[0] for i in range(1500)]
+    b = np.array(rows, dtype=np.uint64)
+    want, _, _ = ol.solve_batch(b, 14, threads=16)
+    got, st = go2.solve(go2.as_boards(b), 14, cfg("mobility", positions_per_wave=ppw, export_min_empties=emin))
+    assert np.array_equal(got, want)
+    assert st["exports"] > 0
+
+
+def test_cross_wavefront_help_deep_batch():
+    b = go2.generate(909, 4, 300, 16)
+    a, _ = go2.solve(b, 16, cfg("mobility", export_min_empties=99))
+    c, st = go2.solve(b, 16, cfg("mobility", positions_per_wave=64, export_min_empties=11))
+    assert np.array_equal(a, c)
+    assert st["exports"] > 0
