@@ -49,12 +49,21 @@ def main() -> int:
     ap.add_argument("--empties", type=int, default=16)
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--ordering", choices=["best", "mobility", "learned", "static"], default="best")
-    ap.add_argument("--plain-n", type=int, default=512, help="positions for the plain-kernel leg (0 = skip)")
+    ap.add_argument("--plain-n", type=int, default=256, help="positions for the plain-kernel leg (0 = skip)")
     ap.add_argument("--plain-limit-s", type=int, default=150, help="time limit of the plain leg")
     ap.add_argument("--cpu-sample", type=int, default=192, help="positions for the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only copy BYTES device-to-device (HBM counter calibration for rocprofv3)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        from gpuothello2_amd.hip import DeviceBuffer, hip
+        src, dst = DeviceBuffer(args.copy_calibration), DeviceBuffer(args.copy_calibration)
+        for _ in range(3):
+            assert hip().hipMemcpy(dst.ptr, src.ptr, args.copy_calibration, 3) == 0
+        print(json.dumps({"copy_bytes": args.copy_calibration, "copies": 3}))
+        return 0
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -116,6 +125,7 @@ def main() -> int:
     # same prefix (same batch size) for the ratio.
     pn = min(args.plain_n, n)
     plain_value = None
+    plain_bound = None
     plain_match = None
     head_small_value = None
     if pn > 0:
@@ -131,6 +141,7 @@ def main() -> int:
         except go2.Go2Error as e:
             log(f"[bench] plain leg did not finish: {e}")
             plain_scores = None
+            plain_bound = pn / args.plain_limit_s  # it took longer than the limit
         pt.close()
         ht = go2.Task(pn, E, cfg)
         ht.upload(boards[:pn])
@@ -162,6 +173,11 @@ def main() -> int:
                          f"{threads} threads, {secs:.1f} s"}
         log(f"[bench] cpu baseline {k / secs:.1f} pos/s on {threads} threads; match={cpu_match}")
 
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "solve_kernel_16e_traffic.json")
+    if os.path.exists(tpath) and n == 100000 and E == 16:
+        with open(tpath) as f:
+            traffic = json.load(f)["bytes_per_launch"]  # rocprofv3 FETCH_SIZE/WRITE_SIZE of this step
     line = {
         "metric": "exact-solved positions/sec @16 empties, 1 GPU (learned-order vs plain), 100% match",
         "value": value,
@@ -181,6 +197,8 @@ def main() -> int:
             "plain_positions": pn, "plain_value": plain_value,
             "headline_value_same_sample": head_small_value,
             "ratio_vs_plain_same_sample": (head_small_value / plain_value) if plain_value else None,
+            "plain_value_upper_bound": plain_bound,
+            "ratio_vs_plain_lower_bound": (head_small_value / plain_bound) if plain_bound else None,
             "score_match_plain": plain_match, "score_match_cpu": cpu_match,
             "nodes_per_position": nodes_per_launch / n,
             "evaluator": None if learned_meta is None else {
@@ -189,7 +207,7 @@ def main() -> int:
         },
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_INT32_TOPS,
                      "unit": "Tops/s (int32 lane-ops)", "frac": achieved / PEAK_INT32_TOPS,
-                     "traffic": None, "kernel_ms": k_ms,
+                     "traffic": traffic, "kernel_ms": k_ms,
                      "ops_per_node": ALGO_INT_OPS_PER_NODE},
         "cpu_baseline": cpu,
     }

@@ -1,0 +1,160 @@
+"""The five workload configurations of this solver, run end to end on one GPU.
+
+  C1  10,000 positions at 10 empties, plain reference method on the host CPU
+      (the oracle's reference-method search), and the same set on the GPU.
+  C2  1,000,000 positions at 12 empties, plain batched GPU kernel (static
+      corner / X-square ordering, no table); bit-exact against the CPU on a
+      sample.
+  C3  200,000 positions with empties drawn uniformly from 10..18, one
+      persistent launch per solve replayed from a captured HIP graph, launch
+      configuration picked by the autotuner on a sample.
+  C4  learned ordering: the trained int16 evaluator orders shallow nodes of
+      100,000 positions at 16 empties; the same positions with mobility-only
+      ordering; node counts and positions/s for both.
+  C5  8,192 positions at 20 empties with a 2^26-bucket transposition table
+      and learned ordering; hit rate and table traffic.
+
+Every configuration prints one JSON line.  Positions come from the GPU
+generator (random legal playouts, SplitMix64 streams).  Usage:
+
+    python -m gpuothello2_amd.workloads [c1 c2 c3 c4 c5] [--scale 0.1]
+
+`--scale` shrinks every batch (for quick checks); the configurations above
+are scale 1.
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import (ORDER_LEARNED, ORDER_MOBILITY, ORDER_STATIC, Task, default_config, generate)
+from . import autotune, learn
+
+
+def log(msg: str) -> None:
+    sys.stderr.write(msg + "\n")
+    sys.stderr.flush()
+
+
+def _oracle():
+    here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    sys.path.insert(0, os.path.join(here, "tests"))
+    import oracle_lib  # test infrastructure: the CPU checker
+
+    return oracle_lib
+
+
+def _solve(boards, depth, cfg, label):
+    t = Task(len(boards), depth, cfg)
+    t.upload(boards)
+    t0 = time.perf_counter()
+    t.launch()
+    t.wait()
+    wall = time.perf_counter() - t0
+    s, st = t.download()
+    t.close()
+    log(f"[{label}] {len(boards)} positions: kernel {st['kernel_ms']:.0f} ms, nodes {st['nodes']}")
+    return s, st, wall
+
+
+def _pairs(b):
+    return np.stack([b["player"], b["opponent"]], 1)
+
+
+def c1(scale: float) -> dict:
+    ol = _oracle()
+    n = max(100, int(10000 * scale))
+    b = generate(101, 1, n, 10)
+    cpu, nodes, secs = ol.solve_batch(_pairs(b), 10, threads=min(16, os.cpu_count() or 1))
+    g, st, _ = _solve(b, 10, default_config(ordering=ORDER_STATIC), "c1")
+    return {"config": "C1", "positions": n, "empties": 10, "cpu_positions_per_s": n / secs,
+            "cpu_threads": min(16, os.cpu_count() or 1), "gpu_positions_per_s": n / (st["kernel_ms"] / 1e3),
+            "match": bool(np.array_equal(cpu, g))}
+
+
+def c2(scale: float) -> dict:
+    ol = _oracle()
+    n = max(1000, int(1000000 * scale))
+    b = generate(202, 2, n, 12)
+    s, st, _ = _solve(b, 12, default_config(ordering=ORDER_STATIC), "c2")
+    k = min(n, 20000)
+    cpu, _, _ = ol.solve_batch(_pairs(b[:k]), 12, threads=min(16, os.cpu_count() or 1))
+    return {"config": "C2", "positions": n, "empties": 12, "ordering": "static",
+            "positions_per_s": n / (st["kernel_ms"] / 1e3), "nodes_per_position": st["nodes"] / n,
+            "checked_vs_cpu": k, "match": bool(np.array_equal(cpu, s[:k]))}
+
+
+def c3(scale: float) -> dict:
+    ol = _oracle()
+    n = max(2000, int(200000 * scale))
+    b = generate(303, 3, n, 10, 18)
+    base = default_config(ordering=ORDER_LEARNED if os.path.exists(learn.WEIGHTS_PATH) else ORDER_MOBILITY)
+    if base.ordering == ORDER_LEARNED:
+        learn.install()
+    tuned = autotune.autotune(b, 18, base, sample=max(1000, n // 10), log=log)
+    cfg = autotune.apply(base, tuned["params"])
+    cfg.graph_launch = 1
+    t = Task(n, 18, cfg)
+    t.upload(b)
+    times = []
+    for _ in range(2):  # first launch captures the graph, the second replays it
+        t.launch()
+        t.wait()
+        s, st = t.download()
+        times.append(st["kernel_ms"])
+    t.close()
+    e = 64 - np.array([bin(int(p) | int(o)).count("1") for p, o in zip(b["player"], b["opponent"])])
+    low = np.nonzero(e <= 14)[0][:5000]
+    cpu, _, _ = ol.solve_batch(_pairs(b[low]), 14, threads=min(16, os.cpu_count() or 1))
+    return {"config": "C3", "positions": n, "empties": "10..18",
+            "histogram": {int(k): int(v) for k, v in zip(*np.unique(e, return_counts=True))},
+            "tuned": tuned["params"], "graph_replay_ms": times[1], "capture_launch_ms": times[0],
+            "positions_per_s": n / (times[1] / 1e3), "checked_vs_cpu": int(len(low)),
+            "match": bool(np.array_equal(cpu, s[low]))}
+
+
+def c4(scale: float) -> dict:
+    n = max(2000, int(100000 * scale))
+    meta = learn.install()
+    b = generate(20261203, 4, n, 16)
+    sl, stl, _ = _solve(b, 16, default_config(ordering=ORDER_LEARNED), "c4 learned")
+    sm, stm, _ = _solve(b, 16, default_config(ordering=ORDER_MOBILITY), "c4 mobility")
+    return {"config": "C4", "positions": n, "empties": 16,
+            "learned": {"positions_per_s": n / (stl["kernel_ms"] / 1e3), "nodes_per_position": stl["nodes"] / n},
+            "mobility_only": {"positions_per_s": n / (stm["kernel_ms"] / 1e3), "nodes_per_position": stm["nodes"] / n},
+            "evaluator_valid_mse": meta["valid_mse"], "match": bool(np.array_equal(sl, sm))}
+
+
+def c5(scale: float) -> dict:
+    n = max(256, int(8192 * scale))
+    learn.install()
+    b = generate(505, 5, n, 20)
+    cfg = default_config(ordering=ORDER_LEARNED, tt_log2_buckets=26, tt_min_empties=10, time_limit_ms=0)
+    s, st, _ = _solve(b, 20, cfg, "c5")
+    secs = st["kernel_ms"] / 1e3
+    k = min(n, 256)
+    cfg2 = default_config(ordering=ORDER_LEARNED, time_limit_ms=0)
+    s2, st2, _ = _solve(b[:k], 20, cfg2, "c5 no table (prefix)")
+    return {"config": "C5", "positions": n, "empties": 20, "table_buckets": 2 ** 26,
+            "positions_per_s": n / secs, "nodes_per_position": st["nodes"] / n,
+            "table_probes": st["tt_probes"], "table_hits": st["tt_hits"], "table_cuts": st["tt_cuts"],
+            "table_stores": st["tt_stores"], "hit_rate": st["tt_hits"] / max(1, st["tt_probes"]),
+            "table_gb_per_s_algorithmic": (st["tt_probes"] + st["tt_stores"]) * 16 / secs / 1e9,
+            "no_table_prefix_match": bool(np.array_equal(s[:k], s2))}
+
+
+RUNNERS = {"c1": c1, "c2": c2, "c3": c3, "c4": c4, "c5": c5}
+
+if __name__ == "__main__":
+    args = [a for a in sys.argv[1:] if not a.startswith("--")]
+    scale = 1.0
+    if "--scale" in sys.argv:
+        scale = float(sys.argv[sys.argv.index("--scale") + 1])
+        args = [a for a in args if a != sys.argv[sys.argv.index("--scale") + 1]]
+    for name in args or list(RUNNERS):
+        log(f"[workloads] {name} (scale {scale})")
+        print(json.dumps(RUNNERS[name](scale)), flush=True)
