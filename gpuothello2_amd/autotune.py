@@ -25,7 +25,7 @@ from . import Config, Go2Error, Task, default_config
 SPACE = {
     "positions_per_wave": [0, 64, 16, 4],
     "block_size": [256, 128],
-    "lds_levels": [9, 7, 6],
+    "lds_levels": [6, 5, 9],
     "split_min_empties": [8, 7, 10],
 }
 DEEP_SPACE = {"tt_log2_buckets": [0, 22, 26]}

@@ -38,8 +38,8 @@ int resolve_config(const go2_config* cfg, go2_config* out) {
   }
   if (c.ordering < GO2_ORDER_STATIC || c.ordering > GO2_ORDER_LEARNED)
     return fail(GO2_E_ARG, "config.ordering must be GO2_ORDER_STATIC, _MOBILITY or _LEARNED");
-  if (c.block_size < 64 || c.block_size > 1024 || c.block_size % 64)
-    return fail(GO2_E_ARG, "config.block_size must be a multiple of 64 in [64, 1024]");
+  if (c.block_size < 64 || c.block_size > 256 || c.block_size % 64)
+    return fail(GO2_E_ARG, "config.block_size must be a multiple of 64 in [64, 256]");
   if (c.mobility_min_empties < 3) c.mobility_min_empties = 3;
   if (c.learned_min_empties < 3) c.learned_min_empties = 3;
   if (c.blocks_per_cu < 0) return fail(GO2_E_ARG, "config.blocks_per_cu must be >= 0");

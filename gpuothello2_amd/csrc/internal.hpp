@@ -25,7 +25,12 @@ struct SolveCounters {
   unsigned long long tt_cuts;
   unsigned long long tt_stores;
   unsigned int tt_gen;  // launch generation 1..255 (advanced by the reset kernel, never zeroed)
-  unsigned int active_waves;  // wavefronts not idle (cross-wavefront help)
+  // Cross-wavefront help.  A wavefront counts itself active when it starts
+  // (not at launch: a launched wavefront need not be resident yet) and while
+  // it holds work; idle ones poll for exported siblings and leave once no
+  // wavefront is active.
+  unsigned int active_waves;
+  unsigned int idle_waves;  // wavefronts polling for exported siblings
   unsigned long long exports;
 };
 

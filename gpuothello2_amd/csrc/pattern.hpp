@@ -121,33 +121,60 @@ __host__ __device__ __forceinline__ uint32_t gather_dyn(int k, uint64_t x) {
   }
 }
 
+// Symmetries visited in Gray-code order, so that each step is one cheap
+// transform of the boards seen so far: toggling the column mirror before a
+// transpose equals mirroring rows after it, and vice versa.
+constexpr int kGray[8] = {0, 1, 3, 2, 6, 7, 5, 4};
+enum { kStepNone = 0, kStepCols = 1, kStepRows = 2, kStepTranspose = 3 };
+constexpr int kGrayStep[8] = {kStepNone, kStepCols, kStepRows, kStepCols,
+                              kStepTranspose, kStepRows, kStepCols, kStepRows};
+// Classes with an instance under symmetry s (bit k = class k), and the
+// instance number of (class k, symmetry s) in kInstances order.
+constexpr int kSymClasses[8] = {0x7FF, 0x7C3, 0x7BF, 0x783, 0x3E, 0x3E, 0x2, 0x2};
+constexpr int kInstanceOf[8][kPatClasses] = {
+    {0, 4, 12, 16, 20, 24, 28, 30, 34, 38, 42}, {1, 5, -1, -1, -1, -1, 29, 31, 35, 39, 43},
+    {2, 6, 13, 17, 21, 25, -1, 32, 36, 40, 44}, {3, 7, -1, -1, -1, -1, -1, 33, 37, 41, 45},
+    {-1, 8, 14, 18, 22, 26, -1, -1, -1, -1, -1}, {-1, 9, 15, 19, 23, 27, -1, -1, -1, -1, -1},
+    {-1, 10, -1, -1, -1, -1, -1, -1, -1, -1, -1}, {-1, 11, -1, -1, -1, -1, -1, -1, -1, -1, -1}};
+
 // Calls f(instance, global weight index) for the 46 instances of (P, O).
-// t3[b] = sum_k bit_k(b) * 3^k.  Fully unrolled: symmetries in the order
-// the instance table uses them, each board transform computed once.
+// t3[b] = sum_k bit_k(b) * 3^k.  A loop over the symmetries (uniform across
+// lanes) with one board transform per step keeps code and registers small.
 template <class T3, class F>
 __host__ __device__ __forceinline__ void for_each_feature(uint64_t P, uint64_t O, const T3& t3, F&& f) {
-  uint64_t sp[8], so[8];
-  sp[0] = P;
-  so[0] = O;
-  sp[1] = mirror_cols(P);
-  so[1] = mirror_cols(O);
-  sp[2] = mirror_rows(P);
-  so[2] = mirror_rows(O);
-  sp[3] = mirror_rows(sp[1]);
-  so[3] = mirror_rows(so[1]);
-  sp[4] = transpose(P);
-  so[4] = transpose(O);
-  sp[5] = transpose(sp[1]);
-  so[5] = transpose(so[1]);
-  sp[6] = transpose(sp[2]);
-  so[6] = transpose(so[2]);
-  sp[7] = transpose(sp[3]);
-  so[7] = transpose(so[3]);
-#pragma unroll
-  for (int i = 0; i < kPatInstances; ++i) {
-    const int k = kInstances[i].cls, s = kInstances[i].sym;
-    const uint32_t a = gather_dyn(k, sp[s]), b = gather_dyn(k, so[s]);
-    f(i, (uint32_t)kClassOffset[k] + (uint32_t)t3[a] + 2u * (uint32_t)t3[b]);
+  uint64_t xp = P, xo = O;
+#pragma unroll 1
+  for (int i = 0; i < 8; ++i) {
+    const int st = kGrayStep[i];
+    if (st == kStepCols) {
+      xp = mirror_cols(xp);
+      xo = mirror_cols(xo);
+    } else if (st == kStepRows) {
+      xp = mirror_rows(xp);
+      xo = mirror_rows(xo);
+    } else if (st == kStepTranspose) {
+      xp = transpose(xp);
+      xo = transpose(xo);
+    }
+    const int s = kGray[i];
+    const int m = kSymClasses[s];
+#define GO2_PAT_CLASS(K)                                                                        \
+    if (m & (1 << K)) {                                                                        \
+      const uint32_t a = gather<K>(xp), b = gather<K>(xo);                                     \
+      f(kInstanceOf[s][K], (uint32_t)kClassOffset[K] + (uint32_t)t3[a] + 2u * (uint32_t)t3[b]); \
+    }
+    GO2_PAT_CLASS(0)
+    GO2_PAT_CLASS(1)
+    GO2_PAT_CLASS(2)
+    GO2_PAT_CLASS(3)
+    GO2_PAT_CLASS(4)
+    GO2_PAT_CLASS(5)
+    GO2_PAT_CLASS(6)
+    GO2_PAT_CLASS(7)
+    GO2_PAT_CLASS(8)
+    GO2_PAT_CLASS(9)
+    GO2_PAT_CLASS(10)
+#undef GO2_PAT_CLASS
   }
 }
 

@@ -159,7 +159,6 @@ struct SolveArgs {
   unsigned int* mb_done;
   unsigned int* mb_cut;
   unsigned int* mb_slot;  // slot index << 6 | move square of the frame's exported sibling
-  unsigned int grid_waves;
   SolveCounters* ctr;
   // HBM frames of levels >= lds_levels+3: [level - lds_levels - 3][total_lanes]
   uint64_t* g_fb;
@@ -266,7 +265,7 @@ __device__ __forceinline__ void undo_move(uint64_t& P, uint64_t& O, uint64_t fb,
 }
 
 template <int MAXE, int ORDER>
-__global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
+__global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
   extern __shared__ uint64_t lds[];
   const int bs = blockDim.x, tid = threadIdx.x;
   const int lane = __lane_id();
@@ -432,6 +431,7 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
   const unsigned int tt_gen = a.tt ? *a.tt_gen : 0u;
   unsigned int trip = 0;
   const unsigned long long deadline = __builtin_amdgcn_s_memrealtime() + a.limit_ticks;
+  if (a.slots && lane == 0) atomicAdd(&a.ctr->active_waves, 1u);
   for (;;) {
     // ---- time limit (checked every 256 trips; uniform per wavefront) ----
     ++trip;
@@ -504,7 +504,10 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
       // an exported sibling always has its active owner waiting for it).
       if (!wave_idle) {
         wave_idle = true;
-        if (lane == 0) atomicSub(&a.ctr->active_waves, 1u);
+        if (lane == 0) {
+          atomicAdd(&a.ctr->idle_waves, 1u);
+          atomicSub(&a.ctr->active_waves, 1u);
+        }
       }
       const unsigned int sidx = (poll_base + (unsigned int)lane) % a.nslots;
       poll_base += 64;
@@ -516,7 +519,10 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
       if (__ballot(won)) {
-        if (lane == 0) atomicAdd(&a.ctr->active_waves, 1u);
+        if (lane == 0) {
+          atomicAdd(&a.ctr->active_waves, 1u);
+          atomicSub(&a.ctr->idle_waves, 1u);
+        }
         wave_idle = false;
         if (won) {
           HelpSlot* hs = &a.slots[sidx];
@@ -532,7 +538,10 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
         unsigned int act = 0;
         if (lane == 0) act = ld_agent(&a.ctr->active_waves);
         act = (unsigned int)__builtin_amdgcn_readlane((int)act, 0);
-        if (act == 0) break;
+        if (act == 0) {
+          if (lane == 0) atomicSub(&a.ctr->idle_waves, 1u);
+          break;
+        }
         __builtin_amdgcn_s_sleep(16);
       }
     }
@@ -587,10 +596,7 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
         dead_new = true;
         split_mask = 0;
       }
-      if (lane == 0) {
-        const unsigned int act = ld_agent(&a.ctr->active_waves);
-        demand = act < a.grid_waves;
-      }
+      if (lane == 0) demand = ld_agent(&a.ctr->idle_waves) != 0;
       demand = __builtin_amdgcn_readlane((int)demand, 0);
     }
     uint64_t rm = __ballot(mode == REPORT && !owner_ext);
@@ -1010,8 +1016,9 @@ __global__ void __launch_bounds__(256) solve_kernel(SolveArgs a) {
   }
 }
 
-__global__ void reset_counters_kernel(SolveCounters* c, unsigned int waves) {
-  c->active_waves = waves;
+__global__ void reset_counters_kernel(SolveCounters* c) {
+  c->active_waves = 0;
+  c->idle_waves = 0;
   c->exports = 0;
   c->tt_gen = c->tt_gen % 255u + 1u;
   c->next = 0;
@@ -1186,10 +1193,10 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   const int maxe = stack_levels_for(max_depth, &fn, cfg.ordering);
   const int bs = cfg.block_size;
   const int levels = maxe - 2;  // frame levels 3..maxe
-  // Keep the stack in LDS up to 11 empties (two 256-lane blocks per CU with
-  // the last-square and base-3 tables); only nodes nearer the root than that
-  // use HBM frames.
-  int lds_levels = cfg.lds_levels > 0 ? cfg.lds_levels : 9;
+  // Keep the stack in LDS up to 8 empties: three 256-lane blocks per CU (the
+  // kernel's register budget allows three wavefronts per SIMD) with the
+  // last-square and base-3 tables; nodes nearer the root use HBM frames.
+  int lds_levels = cfg.lds_levels > 0 ? cfg.lds_levels : 6;
   if (lds_levels > levels) lds_levels = levels;
   const size_t lds = (size_t)lds_levels * bs * 32 + 2048 + (cfg.ordering == GO2_ORDER_LEARNED ? kBase3Size * 2 : 0);
   hipError_t e;
@@ -1266,7 +1273,6 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   } else {
     args.ext_min = 3;  // keeps the (disabled) export test's shift in range
   }
-  args.grid_waves = (unsigned int)waves;
   if (cfg.tt_log2_buckets > 0) {
     if ((rc = ws.ensure_tt(cfg.tt_log2_buckets, stream)) != GO2_OK) return rc;
     if ((rc = zobrist_table(dev, &args.zob)) != GO2_OK) return rc;
@@ -1309,7 +1315,7 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   ws.grid_waves = waves;
 
   auto enqueue = [&]() -> int {
-    reset_counters_kernel<<<1, 1, 0, stream>>>(ws.ctr, (unsigned int)waves);
+    reset_counters_kernel<<<1, 1, 0, stream>>>(ws.ctr);
     hipError_t le = hipGetLastError();
     if (le != hipSuccess) return hip_fail(le, "reset_counters launch");
     if (args.slots) {

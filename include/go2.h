@@ -54,7 +54,7 @@ typedef struct go2_config {
   int32_t ordering;             /* GO2_ORDER_* */
   int32_t mobility_min_empties; /* fastest-first at nodes with >= this many empties */
   int32_t learned_min_empties;  /* evaluator ordering at nodes with >= this many empties */
-  int32_t block_size;           /* lanes per workgroup, multiple of 64, 64..1024 */
+  int32_t block_size;           /* lanes per workgroup, multiple of 64, 64..256 */
   int32_t blocks_per_cu;        /* resident workgroups per CU to launch; 0 = as many as fit */
   int32_t positions_per_wave;   /* boards a wavefront owns at once (1..64); 0 = auto from n.
                                    Lanes without a board of their own help the others. */

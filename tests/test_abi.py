@@ -63,6 +63,9 @@ def test_argument_errors_need_no_device():
         go2.solve(b, 12, go2.default_config(block_size=100))
     assert e.value.code == go2.GO2_E_ARG
     with pytest.raises(go2.Go2Error) as e:
+        go2.solve(b, 12, go2.default_config(block_size=512))
+    assert e.value.code == go2.GO2_E_ARG
+    with pytest.raises(go2.Go2Error) as e:
         go2.solve(b, 12, go2.default_config(ordering=7))
     assert e.value.code == go2.GO2_E_ARG
     L = go2.lib()

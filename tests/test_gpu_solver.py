@@ -211,6 +211,20 @@ def test_cross_wavefront_help_keeps_scores_exact(ppw, emin):
     assert st["exports"] > 0
 
 
+# Wavefronts count themselves active when they start, so idle ones leave even
+# if part of the launched grid is not resident at the same time.  Odd wave
+# counts per block (3) and small LDS stacks give grids of 4 blocks per CU.
+@pytest.mark.parametrize("bs,ll", [(192, 6), (192, 4), (64, 2)])
+@pytest.mark.parametrize("ordering", ["static", "mobility"])
+def test_full_grid_any_block_shape(bs, ll, ordering):
+    b = np.concatenate([ol.generate(611, 2, 3000, 12), ol.generate(612, 2, 500, 11)])
+    want, _, _ = ol.solve_batch(b, 12, threads=16)
+    got, st = go2.solve(go2.as_boards(b), 12,
+                        cfg(ordering, block_size=bs, lds_levels=ll, time_limit_ms=60000))
+    print(f"block {bs} lds {ll} {ordering}: grid_waves {st['grid_waves']}", flush=True)
+    assert np.array_equal(got, want)
+
+
 def test_cross_wavefront_help_deep_batch():
     b = go2.generate(909, 4, 300, 16)
     a, _ = go2.solve(b, 16, cfg("mobility", export_min_empties=99))
