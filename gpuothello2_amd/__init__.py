@@ -75,7 +75,8 @@ class Config(ctypes.Structure):
         ("tt_min_empties", ctypes.c_int32),
         ("graph_launch", ctypes.c_int32),
         ("export_min_empties", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 7),
+        ("export_low_empties", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 6),
     ]
 
 

@@ -48,6 +48,8 @@ int resolve_config(const go2_config* cfg, go2_config* out) {
   if (c.split_min_empties < 0) return fail(GO2_E_ARG, "config.split_min_empties must be >= 0");
   if (c.lds_levels < 0 || c.lds_levels > 30) return fail(GO2_E_ARG, "config.lds_levels must be in [0, 30]");
   if (c.time_limit_ms < 0) return fail(GO2_E_ARG, "config.time_limit_ms must be >= 0");
+  if (c.export_min_empties < 0) return fail(GO2_E_ARG, "config.export_min_empties must be >= 0");
+  if (c.export_low_empties < 0) return fail(GO2_E_ARG, "config.export_low_empties must be >= 0");
   if (c.tt_log2_buckets < 0 || c.tt_log2_buckets > 32)
     return fail(GO2_E_ARG, "config.tt_log2_buckets must be in [0, 32]");
   if (c.split_cut_after < 0 || c.split_cut_after > 3)

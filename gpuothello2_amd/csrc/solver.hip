@@ -148,17 +148,21 @@ struct SolveArgs {
   int tt_min;
   const unsigned int* tt_gen;  // launch generation (device, set by the reset kernel)
   const uint64_t* zob;  // Zobrist keys per (byte position 0..15, byte value)
-  // cross-wavefront help (null = off): once the board queue is empty, busy
-  // wavefronts export siblings of nodes with >= ext_min empties to `slots`;
-  // idle wavefronts claim them and report through per-frame mailboxes
-  // [E - ext_min][total_lanes]
+  // cross-wavefront help (null = off): while some wavefront is idle, busy
+  // wavefronts export siblings to `slots`: of nodes with >= ext_min empties
+  // when few wavefronts are idle, down to ext_lo empties when nearly all
+  // are; idle wavefronts claim them and report through per-frame mailboxes
+  // [E - ext_lo][total_lanes]
   struct HelpSlot* slots;
   unsigned int nslots;
   int ext_min;
+  int ext_lo;
   int* mb_best;
   unsigned int* mb_done;
   unsigned int* mb_cut;
-  unsigned int* mb_slot;  // slot index << 6 | move square of the frame's exported sibling
+  // [E - ext_lo][kExtRec][total_lanes]: 1 + (slot index << 6 | move square)
+  // of a sibling that may still be unclaimed, 0 = none
+  unsigned int* mb_rec;
   SolveCounters* ctr;
   // HBM frames of levels >= lds_levels+3: [level - lds_levels - 3][total_lanes]
   uint64_t* g_fb;
@@ -167,6 +171,8 @@ struct SolveArgs {
   uint32_t* g_wd;
   uint32_t* g_sx;
 };
+// The graph key holds kernel, grid, block, LDS bytes and these arguments.
+static_assert(sizeof(SolveArgs) + 4 * sizeof(size_t) <= 512, "graph key too small");
 
 // Split word of a frame (levels >= split_min): helpers still out on this
 // node (bits 0-7) and the best value they reported, halved and offset by 64
@@ -228,11 +234,11 @@ struct HelpSlot {
   unsigned int pad[8];
 };
 constexpr unsigned int kSlotEmpty = 0, kSlotWriting = 1, kSlotFull = 2, kSlotTaken = 3;
-// A full slot carries its owner's tag (global lane id and frame level), so
-// an owner taking back an unclaimed sibling can never cancel another
-// owner's item that reused the slot.
-__device__ __forceinline__ unsigned int slot_full(unsigned int g, int e) {
-  return ((g << 5 | (unsigned int)e) << 2) | kSlotFull;
+// A full slot carries its owner's tag (global lane id < 2^19, frame level
+// and move square), so an owner taking back an unclaimed sibling can never
+// cancel another item that reused the slot, its own included.
+__device__ __forceinline__ unsigned int slot_full(unsigned int g, int e, int sq) {
+  return ((((g << 5 | (unsigned int)e) << 6) | (unsigned int)sq) << 2) | kSlotFull;
 }
 
 template <class T>
@@ -244,10 +250,12 @@ __device__ __forceinline__ void st_agent(T* p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Split word bit 29: a sibling of this node is out with another wavefront
-// (at most one per node; its owner takes it back if nobody claims it).
-__device__ __forceinline__ int sx_ext(uint32_t x) { return (int)((x >> 29) & 1u); }
-constexpr uint32_t kSxExt = 1u << 29;
+// Split word bits 29-31: siblings of this node out with other wavefronts
+// (up to kExtRec; not yet reported, or unclaimed and recorded so that the
+// owner can take them back).
+__device__ __forceinline__ int sx_ext(uint32_t x) { return (int)(x >> 29); }
+constexpr uint32_t kSxExtOne = 1u << 29;
+constexpr int kExtRec = 4;
 
 // Board of the parent of (P, O), from the frame word and flip set the parent
 // stored when it moved to (P, O).
@@ -351,6 +359,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
   uint32_t ext_mask = 0;        // bit E-3: frame E has siblings out with other wavefronts
   bool wave_idle = false;       // wavefront-uniform: counted out of active_waves
   bool demand = false;          // wavefront-uniform: some wavefront is idle
+  int dem_e = 64;               // wavefront-uniform: lowest level to export from
   unsigned int poll_base = gtid - (unsigned int)lane;  // wavefront-uniform
   int alpha0 = 0;               // current node's alpha at entry
   int bestmv = 63;              // move that gave the current node's best value (63 = none)
@@ -364,7 +373,10 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
   bool queue_done = false;      // wavefront-uniform
   unsigned long long c_pos = 0, c_nodes = 0, c_steps = 0, c_iters = 0, c_steals = 0, c_slots = 0;
   unsigned long long c_probes = 0, c_hits = 0, c_cuts = 0, c_stores = 0, c_exports = 0;
-  auto mb_idx = [&](int e, unsigned int g) { return (size_t)(e - a.ext_min) * a.total_lanes + g; };
+  auto mb_idx = [&](int e, unsigned int g) { return (size_t)(e - a.ext_lo) * a.total_lanes + g; };
+  auto rec_idx = [&](int e, int k, unsigned int g) {
+    return ((size_t)(e - a.ext_lo) * kExtRec + (size_t)k) * a.total_lanes + g;
+  };
 
   // A node at level E with legal moves `cm` becomes the current node.
   auto enter_node = [&](uint64_t cm, int type) {
@@ -382,12 +394,6 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
     alpha0 = alpha;
     bestmv = 63;
     if (E >= a.split_min) set_sx(E, sx_init(alpha));
-    if (a.slots && E >= a.ext_min) {
-      const size_t m = mb_idx(E, gtid);
-      st_agent(&a.mb_best[m], kNoValue);
-      st_agent(&a.mb_done[m], 0u);
-      st_agent(&a.mb_cut[m], 0u);
-    }
   };
 
   // This lane starts searching child (cP, cO) of frame oe of its owner, with
@@ -562,31 +568,45 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
           const int e = __builtin_ctz(em) + 3;
           em &= em - 1;
           const size_t m = mb_idx(e, gtid);
+          uint32_t sx = get_sx(e);
+          const uint32_t sx0 = sx;
           const unsigned int got =
               __hip_atomic_exchange(&a.mb_done[m], 0u, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
           if (got) {
-            uint32_t sx = (get_sx(e) - 1u) & ~kSxExt;
+            sx -= got * (1u + kSxExtOne);
+            helpers_out -= (int)got;
             const int mb = ld_agent(&a.mb_best[m]);
             if (mb >= -64) sx = sx_with_best(sx, mb);
-            set_sx(e, sx);
-            --helpers_out;
-            ext_mask &= ~(1u << (e - 3));
-          } else if ((mode == WAIT && E == e) || mode == ABORT) {
-            // Nobody claimed it while this lane waits: take it back.
-            const unsigned int sl = ld_agent(&a.mb_slot[m]);
-            unsigned int expect = slot_full(gtid, e);
-            if (__hip_atomic_compare_exchange_strong(&a.slots[sl >> 6].state, &expect, kSlotEmpty,
-                                                     __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT)) {
-              set_sx(e, (get_sx(e) - 1u) & ~kSxExt);
-              --helpers_out;
-              ext_mask &= ~(1u << (e - 3));
-              if (mode == WAIT && alpha < beta) {
-                moves |= 1ULL << (sl & 63);
-                mode = SEARCH;
+          }
+          // Waiting on this frame (or aborting): take back every sibling
+          // nobody claimed.  Otherwise, after reports, drop the records of
+          // siblings that were claimed (their slot no longer holds the tag).
+          const bool waiting = mode == WAIT && E == e;
+          const bool take = waiting || mode == ABORT;
+          if (sx_ext(sx) && (take || got)) {
+            for (int k = 0; k < kExtRec; ++k) {
+              const size_t ri = rec_idx(e, k, gtid);
+              const unsigned int r = ld_agent(&a.mb_rec[ri]);
+              if (!r) continue;
+              const unsigned int sl = r - 1u;
+              unsigned int expect = slot_full(gtid, e, (int)(sl & 63));
+              if (take) {
+                if (__hip_atomic_compare_exchange_strong(&a.slots[sl >> 6].state, &expect, kSlotEmpty,
+                                                         __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT)) {
+                  sx -= 1u + kSxExtOne;
+                  --helpers_out;
+                  if (waiting && alpha < beta) moves |= 1ULL << (sl & 63);
+                }
+                st_agent(&a.mb_rec[ri], 0u);
+              } else if (ld_agent(&a.slots[sl >> 6].state) != expect) {
+                st_agent(&a.mb_rec[ri], 0u);
               }
             }
+            if (waiting && alpha < beta && moves) mode = SEARCH;
           }
+          if (sx != sx0) set_sx(e, sx);
+          if (!sx_ext(sx)) ext_mask &= ~(1u << (e - 3));
         }
       }
       if (owner_ext && (mode == SEARCH || mode == WAIT) &&
@@ -596,8 +616,16 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
         dead_new = true;
         split_mask = 0;
       }
-      if (lane == 0) demand = ld_agent(&a.ctr->idle_waves) != 0;
-      demand = __builtin_amdgcn_readlane((int)demand, 0);
+      // Export level: ext_min while few wavefronts are idle, falling
+      // linearly to ext_lo as the idle fraction approaches one.
+      if (lane == 0) {
+        const unsigned int iw = ld_agent(&a.ctr->idle_waves);
+        const unsigned int gw = gridDim.x * (blockDim.x >> 6);
+        dem_e = iw ? a.ext_min - (int)(((unsigned long long)(a.ext_min - a.ext_lo + 1) * iw) / (gw + 1u)) : 64;
+        if (dem_e < a.ext_lo) dem_e = a.ext_lo;
+      }
+      dem_e = __builtin_amdgcn_readlane(dem_e, 0);
+      demand = dem_e < 64;
     }
     uint64_t rm = __ballot(mode == REPORT && !owner_ext);
     while (rm) {
@@ -697,14 +725,14 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
           start_child_task(cP, cO, ca, cb, ct, victim, e, false);
         }
       }
-    } else if (a.slots && demand && !idle && __ballot((split_mask >> (a.ext_min - 3)) != 0)) {
+    } else if (a.slots && demand && !idle && __ballot((split_mask >> (dem_e - 3)) != 0)) {
       // No idle lane here but an idle wavefront elsewhere: export the
-      // highest offered sibling (>= ext_min empties) to a help slot.
-      uint32_t lv = split_mask >> (a.ext_min - 3);
+      // highest offered sibling (>= dem_e empties) to a help slot.
+      uint32_t lv = split_mask >> (dem_e - 3);
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) lv |= (uint32_t)__shfl_xor((int)lv, o);
       lv = (uint32_t)__builtin_amdgcn_readfirstlane((int)lv);
-      const int e = 31 - __builtin_clz(lv) + a.ext_min;
+      const int e = 31 - __builtin_clz(lv) + dem_e;
       const int victim = __builtin_ctzll(__ballot((split_mask >> (e - 3)) & 1));
       if (lane == victim) {
         uint64_t mv = get_mv(e), od = get_od(e);
@@ -712,9 +740,15 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
         const uint32_t sx = get_sx(e);
         const int fa = frame_val(w, 0), fbeta = frame_val(w, 7);
         const int am = max(fa, sx_best(sx));
+        // a free record for taking the sibling back (records of claimed
+        // siblings are dropped when their reports arrive)
+        const int cnt = sx_ext(sx);
+        int rk = cnt == 0 ? 0 : kExtRec;
+        for (int k = 0; cnt > 0 && cnt < kExtRec && k < kExtRec; ++k)
+          if (rk == kExtRec && ld_agent(&a.mb_rec[rec_idx(e, k, gtid)]) == 0u) rk = k;
         if (mv == 0 || am >= fbeta) {
           split_mask &= ~(1u << (e - 3));
-        } else if (sx_ext(sx) == 0) {
+        } else if (rk < kExtRec) {
           // claim an empty slot first
           unsigned int sidx = (gtid * 2654435761u + trip) % a.nslots;
           bool got = false;
@@ -738,8 +772,16 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
             const uint64_t bit = 1ULL << sq;
             mv ^= bit;
             set_mv_od(e, mv, od);
-            set_sx(e, (sx + 1u) | kSxExt);
-            st_agent(&a.mb_slot[mb_idx(e, gtid)], (sidx << 6) | (unsigned int)sq);
+            if (cnt == 0) {
+              // first sibling out from this node: fresh mailbox and records
+              const size_t m = mb_idx(e, gtid);
+              st_agent(&a.mb_best[m], kNoValue);
+              st_agent(&a.mb_done[m], 0u);
+              st_agent(&a.mb_cut[m], 0u);
+              for (int k = 1; k < kExtRec; ++k) st_agent(&a.mb_rec[rec_idx(e, k, gtid)], 0u);
+            }
+            set_sx(e, sx + 1u + kSxExtOne);
+            st_agent(&a.mb_rec[rec_idx(e, rk, gtid)], 1u + ((sidx << 6) | (unsigned int)sq));
             ++helpers_out;
             ext_mask |= 1u << (e - 3);
             if (!mv) split_mask &= ~(1u << (e - 3));
@@ -752,7 +794,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
             st_agent(&hs->ctype, child_type((int)(w >> 30), 1));
             st_agent(&hs->owner_E, e);
             st_agent(&hs->owner, gtid);
-            __hip_atomic_store(&hs->state, slot_full(gtid, e), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&hs->state, slot_full(gtid, e, sq), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             ++c_exports;
           }
         }
@@ -874,8 +916,14 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
       }
       if (push) {
         push_frame(E, f | bit, moves, order, pack_frame(alpha, beta, best, sq, passed, ndone, ntype));
-        if (E >= a.split_min && moves && ndone >= (ntype == kCut ? a.cut_after : 1))
-          split_mask |= 1u << (E - 3);
+        // Young-brothers-wait: siblings are offered after the first child
+        // (after cut_after children at expected CUT nodes).  When nearly
+        // every wavefront is idle, speculation costs little and the serial
+        // chain of the last boards is what remains: offer after one child
+        // at CUT nodes and at once at ALL nodes.
+        const bool tail = dem_e <= a.ext_lo;
+        const int thr = ntype == kCut ? (tail ? 1 : a.cut_after) : (tail && ntype == kAll ? 0 : 1);
+        if (E >= a.split_min && moves && ndone >= thr) split_mask |= 1u << (E - 3);
         const int ct = child_type(ntype, ndone);
         if (a.tt && E >= a.tt_min) set_sx(E, (get_sx(E) & ~(63u << 23)) | ((uint32_t)bestmv << 23));
         if (passed) {
@@ -1215,10 +1263,18 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   if (cfg.blocks_per_cu > 0 && cfg.blocks_per_cu < per_cu) per_cu = cfg.blocks_per_cu;
   const size_t waves_per_block = (size_t)bs / 64;
   size_t grid = (size_t)cus * per_cu;
+  // Export levels: ext_min (default 12) while few wavefronts are idle, down
+  // to ext_lo (default 9) when nearly all are; never below split_min.
+  int split_min = cfg.split_min_empties > 0 ? cfg.split_min_empties : 8;
+  if (split_min < 5) split_min = 5;
+  if (split_min > maxe + 1) split_min = maxe + 1;  // no split levels at all
+  const int ext_min = cfg.export_min_empties > 0 ? max(cfg.export_min_empties, split_min) : max(12, split_min);
+  const int ext_lo = min(ext_min, max(split_min, cfg.export_low_empties > 0 ? cfg.export_low_empties : 9));
   // Without cross-wavefront help, never launch more wavefronts than boards;
   // with it, the whole resident grid runs and idle wavefronts take siblings.
-  const bool help = cfg.export_min_empties <= 60 &&
-                    (cfg.export_min_empties > 0 ? max(cfg.export_min_empties, 5) : 12) <= maxe;
+  // Slot tags hold global lane ids below 2^19.
+  const bool help = cfg.export_min_empties <= 60 && ext_lo <= maxe &&
+                    (size_t)cus * per_cu * bs < (size_t(1) << 19);
   const size_t max_grid = (n + waves_per_block - 1) / waves_per_block;
   if (!help && grid > max_grid) grid = max_grid;
   if (grid < 1) grid = 1;
@@ -1227,9 +1283,6 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   if (K <= 0) K = (int)((n + waves - 1) / waves);
   if (K > 64) K = 64;
   if (K < 1) K = 1;
-  int split_min = cfg.split_min_empties > 0 ? cfg.split_min_empties : 8;
-  if (split_min < 5) split_min = 5;
-  if (split_min > maxe + 1) split_min = maxe + 1;  // no split levels at all
 
   const size_t total_lanes = grid * bs;
   const size_t g_levels = (size_t)(levels - lds_levels);
@@ -1251,27 +1304,27 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   args.total_lanes = (unsigned int)total_lanes;
   args.limit_ticks = 0;
   args.max_pops = cfg.max_pops_per_trip > 0 ? cfg.max_pops_per_trip : 2;
-  const int ext_min = cfg.export_min_empties > 0 ? max(cfg.export_min_empties, split_min) : max(12, split_min);
-  if (ext_min <= maxe && cfg.export_min_empties <= 60) {
+  if (help) {
     const unsigned int nslots = 4096;
-    const size_t mlev = (size_t)(maxe + 1 - ext_min);
-    const size_t mb = mlev * total_lanes * 4;
-    const size_t need = (size_t)nslots * sizeof(HelpSlot) + 4 * ((mb + 255) & ~(size_t)255);
+    const size_t mlev = (size_t)(maxe + 1 - ext_lo);
+    const size_t mb = ((mlev * total_lanes * 4) + 255) & ~(size_t)255;
+    const size_t need = (size_t)nslots * sizeof(HelpSlot) + (3 + kExtRec) * mb;
     if ((rc = ws.ensure_help(need, stream)) != GO2_OK) return rc;
     uint8_t* hb = reinterpret_cast<uint8_t*>(ws.help);
     args.slots = reinterpret_cast<HelpSlot*>(hb);
     args.nslots = nslots;
     args.ext_min = ext_min;
+    args.ext_lo = ext_lo;
     size_t o = (size_t)nslots * sizeof(HelpSlot);
     args.mb_best = reinterpret_cast<int*>(hb + o);
-    o += (mb + 255) & ~(size_t)255;
+    o += mb;
     args.mb_done = reinterpret_cast<unsigned int*>(hb + o);
-    o += (mb + 255) & ~(size_t)255;
+    o += mb;
     args.mb_cut = reinterpret_cast<unsigned int*>(hb + o);
-    o += (mb + 255) & ~(size_t)255;
-    args.mb_slot = reinterpret_cast<unsigned int*>(hb + o);
+    o += mb;
+    args.mb_rec = reinterpret_cast<unsigned int*>(hb + o);
   } else {
-    args.ext_min = 3;  // keeps the (disabled) export test's shift in range
+    args.ext_min = args.ext_lo = 3;
   }
   if (cfg.tt_log2_buckets > 0) {
     if ((rc = ws.ensure_tt(cfg.tt_log2_buckets, stream)) != GO2_OK) return rc;

@@ -73,10 +73,13 @@ typedef struct go2_config {
   int32_t tt_min_empties;       /* probe/store nodes with >= this many empties; 0 = auto (12) */
   int32_t graph_launch;         /* tasks capture their launch (counter reset + solve kernel) in a
                                    HIP graph once and replay it on later launches */
-  int32_t export_min_empties;   /* once the board queue is empty, busy wavefronts hand siblings of
+  int32_t export_min_empties;   /* while some wavefront is idle, busy wavefronts hand siblings of
                                    nodes with >= this many empties to idle wavefronts; 0 = auto
                                    (12), > 60 = never */
-  int32_t reserved[7];
+  int32_t export_low_empties;   /* the export level falls linearly from export_min_empties to this
+                                   one as the idle fraction of wavefronts approaches one; 0 = auto
+                                   (9), never below split_min_empties */
+  int32_t reserved[6];
 } go2_config;
 
 /* Counters of one solve.  Replaces the reference's `total` node counter

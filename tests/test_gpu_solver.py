@@ -225,6 +225,20 @@ def test_full_grid_any_block_shape(bs, ll, ordering):
     assert np.array_equal(got, want)
 
 
+# Graded export level: down to `low` empties once most wavefronts are idle,
+# several siblings of one node out with other wavefronts at a time (taken
+# back when nobody claims them).
+@pytest.mark.parametrize("low,smin", [(5, 5), (8, 6), (9, 8)])
+def test_graded_export_keeps_scores_exact(low, smin):
+    rows = [ol.generate(818, 3, 1, int(ol.empties_for(818, 3, i, 10, 14)), start=i)[0] for i in range(600)]
+    b = np.array(rows, dtype=np.uint64)
+    want, _, _ = ol.solve_batch(b, 14, threads=16)
+    got, st = go2.solve(go2.as_boards(b), 14, cfg("mobility", positions_per_wave=2, split_min_empties=smin,
+                                                  export_min_empties=12, export_low_empties=low))
+    assert np.array_equal(got, want)
+    assert st["exports"] > 0
+
+
 def test_cross_wavefront_help_deep_batch():
     b = go2.generate(909, 4, 300, 16)
     a, _ = go2.solve(b, 16, cfg("mobility", export_min_empties=99))
