@@ -30,6 +30,9 @@ GO2_E_PARSE = -6
 GO2_E_TIMEOUT = -7
 GO2_MAX_EMPTIES = 32
 
+MOVE_PASS = 64  # best-move codes besides squares 0..63 (bit 0 = A1)
+MOVE_NONE = 65
+
 ORDER_STATIC = 0
 ORDER_MOBILITY = 1
 ORDER_LEARNED = 2
@@ -38,6 +41,7 @@ ORDER_LEARNED = 2
 EXPORTED_SYMBOLS = (
     "go2_version", "go2_last_error", "go2_set_device", "go2_config_default",
     "go2_board_from_text", "go2_board_to_text", "go2_solve", "go2_solve_device",
+    "go2_solve_moves", "go2_solve_device_moves", "go2_task_download_moves",
     "go2_task_create", "go2_task_upload", "go2_task_launch", "go2_task_query",
     "go2_task_wait", "go2_task_download", "go2_task_destroy", "go2_generate",
     "go2_generate_device", "go2_pattern_info", "go2_pattern_features_device",
@@ -127,6 +131,9 @@ def lib() -> ctypes.CDLL:
     L.go2_board_to_text.restype = None
     L.go2_solve.argtypes = [vp, sz, i32, vp, ctypes.POINTER(Config), ctypes.POINTER(Stats)]
     L.go2_solve_device.argtypes = [vp, sz, i32, vp, ctypes.POINTER(Config), ctypes.POINTER(Stats), vp]
+    L.go2_solve_moves.argtypes = [vp, sz, i32, vp, vp, ctypes.POINTER(Config), ctypes.POINTER(Stats)]
+    L.go2_solve_device_moves.argtypes = [vp, sz, i32, vp, vp, ctypes.POINTER(Config), ctypes.POINTER(Stats), vp]
+    L.go2_task_download_moves.argtypes = [vp, vp]
     L.go2_task_create.argtypes = [sz, i32, ctypes.POINTER(Config)]
     L.go2_task_create.restype = vp
     L.go2_task_upload.argtypes = [vp, vp, sz]
@@ -197,6 +204,30 @@ def solve(boards: np.ndarray, max_depth: int, config: Config | None = None):
     return scores[:n], st.as_dict()
 
 
+def solve_moves(boards: np.ndarray, max_depth: int, config: Config | None = None):
+    """Exact scores and best moves of host boards; returns (int8 scores,
+    uint8 moves, stats dict).  A move is a square 0..63 whose child has the
+    board's score, MOVE_PASS or MOVE_NONE."""
+    b = np.ascontiguousarray(boards, dtype=BOARD_DTYPE)
+    n = b.shape[0]
+    scores = np.zeros(max(n, 1), dtype=np.int8)
+    moves = np.zeros(max(n, 1), dtype=np.uint8)
+    st = Stats()
+    _check(lib().go2_solve_moves(b.ctypes.data if n else None, n, max_depth, scores.ctypes.data,
+                                 moves.ctypes.data, ctypes.byref(config) if config is not None else None,
+                                 ctypes.byref(st)))
+    return scores[:n], moves[:n], st.as_dict()
+
+
+def move_text(m: int) -> str:
+    """The reference's move notation (main.cu:31-39): column letter, row digit; "ps" = pass."""
+    if m == MOVE_PASS:
+        return "ps"
+    if m == MOVE_NONE:
+        return "--"
+    return chr(ord("A") + m % 8) + chr(ord("1") + m // 8)
+
+
 def solve_device(d_boards_ptr: int, n: int, max_depth: int, d_scores_ptr: int,
                  config: Config | None = None, stream: int | None = None) -> dict:
     """Solve boards already in device memory (raw pointers); returns stats."""
@@ -265,6 +296,11 @@ class Task:
         st = Stats()
         _check(lib().go2_task_download(self._h, scores.ctypes.data, ctypes.byref(st)))
         return scores[: self.count], st.as_dict()
+
+    def download_moves(self) -> np.ndarray:
+        moves = np.zeros(max(self.count, 1), dtype=np.uint8)
+        _check(lib().go2_task_download_moves(self._h, moves.ctypes.data))
+        return moves[: self.count]
 
     def close(self) -> None:
         if getattr(self, "_h", None):

@@ -132,6 +132,14 @@ void apply_json_config(const char* path, go2_config* c) {
   }
 }
 
+// Column letter and row digit (bit 0 = A1); "ps" for a pass, "--" when the
+// game is over.
+std::string move_text(int m) {
+  if (m == GO2_MOVE_PASS) return "ps";
+  if (m < 0 || m > 63) return "--";
+  return std::string(1, (char)('A' + m % 8)) + (char)('1' + m / 8);
+}
+
 int env_int(const char* name, int def) {
   const char* s = getenv(name);
   return s && *s ? atoi(s) : def;
@@ -185,6 +193,9 @@ int main(int argc, char** argv) {
     if (go2_set_evaluator(w.data(), b.data(), 64.0f) != GO2_OK) die(2, go2_last_error());
   }
 
+  // GO2_BEST_MOVE=1 adds a best move to each output line, written like the
+  // reference's think-mode moves (main.cu:31-39,114-117).
+  const bool with_moves = env_int("GO2_BEST_MOVE", 0) != 0;
   const size_t n = boards.size();
   fprintf(stderr, "n = %zu\n", n);
   std::vector<go2_task*> tasks;
@@ -211,10 +222,13 @@ int main(int argc, char** argv) {
   }
   const auto t1 = std::chrono::steady_clock::now();
   std::vector<int8_t> scores(n);
+  std::vector<uint8_t> moves(with_moves ? n : 0);
   uint64_t nodes = 0;
   for (size_t k = 0; k < tasks.size(); ++k) {
     go2_stats st;
     if (go2_task_download(tasks[k], scores.data() + k * kBatch, &st) != GO2_OK) die(2, go2_last_error());
+    if (with_moves && go2_task_download_moves(tasks[k], moves.data() + k * kBatch) != GO2_OK)
+      die(2, go2_last_error());
     nodes += st.nodes;
     go2_task_destroy(tasks[k]);
   }
@@ -225,7 +239,10 @@ int main(int argc, char** argv) {
   char buf[17];
   for (size_t j = 0; j < n; ++j) {
     go2_board_to_text(&boards[j], buf);
-    fprintf(out.get(), "%s %d\n", buf, (int)scores[j]);
+    if (with_moves)
+      fprintf(out.get(), "%s %d %s\n", buf, (int)scores[j], move_text(moves[j]).c_str());
+    else
+      fprintf(out.get(), "%s %d\n", buf, (int)scores[j]);
   }
   fprintf(stderr, "total nodes: %" PRIu64 "\n", nodes);
   return 0;

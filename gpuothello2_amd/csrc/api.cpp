@@ -62,15 +62,18 @@ int resolve_config(const go2_config* cfg, go2_config* out) {
 struct IoBufs {
   go2_board* boards = nullptr;
   int8_t* scores = nullptr;
+  uint8_t* moves = nullptr;
   ~IoBufs() {
     if (boards) (void)hipFree(boards);
     if (scores) (void)hipFree(scores);
+    if (moves) (void)hipFree(moves);
   }
   int alloc(size_t n) {
     hipError_t e;
     if ((e = hipMalloc(&boards, n * sizeof(go2_board))) != hipSuccess)
       return hip_fail(e, "hipMalloc(boards)");
     if ((e = hipMalloc(&scores, n)) != hipSuccess) return hip_fail(e, "hipMalloc(scores)");
+    if ((e = hipMalloc(&moves, n)) != hipSuccess) return hip_fail(e, "hipMalloc(moves)");
     return GO2_OK;
   }
 };
@@ -175,6 +178,11 @@ void go2_board_to_text(const go2_board* board, char out[17]) {
 
 int go2_solve_device(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_scores,
                      const go2_config* cfg, go2_stats* stats, void* hip_stream) {
+  return go2_solve_device_moves(d_boards, n, max_depth, d_scores, nullptr, cfg, stats, hip_stream);
+}
+
+int go2_solve_device_moves(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_scores,
+                           uint8_t* d_moves, const go2_config* cfg, go2_stats* stats, void* hip_stream) {
   int rc;
   if ((rc = check_depth(max_depth)) != GO2_OK) return rc;
   if (n && (!d_boards || !d_scores)) return fail(GO2_E_ARG, "null device array");
@@ -188,18 +196,24 @@ int go2_solve_device(const go2_board* d_boards, size_t n, int max_depth, int8_t*
   Workspace ws;
   Events ev;
   if ((rc = ev.create()) != GO2_OK) return rc;
-  if ((rc = launch_solve(d_boards, n, max_depth, d_scores, c, ws, s, ev.a, ev.b)) != GO2_OK) return rc;
+  if ((rc = launch_solve(d_boards, n, max_depth, d_scores, d_moves, c, ws, s, ev.a, ev.b)) != GO2_OK) return rc;
   return collect_stats(ws, s, ev.a, ev.b, stats);
 }
 
 int go2_solve(const go2_board* boards, size_t n, int max_depth, int8_t* scores,
               const go2_config* cfg, go2_stats* stats) {
+  return go2_solve_moves(boards, n, max_depth, scores, nullptr, cfg, stats);
+}
+
+int go2_solve_moves(const go2_board* boards, size_t n, int max_depth, int8_t* scores, uint8_t* moves,
+                    const go2_config* cfg, go2_stats* stats) {
   if (n && (!boards || !scores)) return fail(GO2_E_ARG, "null host array");
   go2_task* t = go2_task_create(n, max_depth, cfg);
   if (!t) return g_last_code;
   int rc = go2_task_upload(t, boards, n);
   if (rc == GO2_OK) rc = go2_task_launch(t);
   if (rc == GO2_OK) rc = go2_task_download(t, scores, stats);
+  if (rc == GO2_OK && moves) rc = go2_task_download_moves(t, moves);
   go2_task_destroy(t);
   return rc;
 }
@@ -246,7 +260,7 @@ int go2_task_launch(go2_task* t) {
   if (!t) return fail(GO2_E_ARG, "null task");
   int rc = GO2_OK;
   if (t->count)
-    rc = launch_solve(t->bufs.boards, t->count, t->max_depth, t->bufs.scores, t->cfg, t->ws,
+    rc = launch_solve(t->bufs.boards, t->count, t->max_depth, t->bufs.scores, t->bufs.moves, t->cfg, t->ws,
                       t->stream, t->ev.a, t->ev.b, t->cfg.graph_launch ? &t->graph : nullptr);
   if (rc == GO2_OK) t->launched = true;
   return rc;
@@ -281,6 +295,19 @@ int go2_task_download(go2_task* t, int8_t* scores, go2_stats* stats) {
   if (rc != GO2_OK) return rc;
   hipError_t e = hipMemcpyAsync(scores, t->bufs.scores, t->count, hipMemcpyDeviceToHost, t->stream);
   if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(scores)");
+  if ((e = hipStreamSynchronize(t->stream)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+  return GO2_OK;
+}
+
+int go2_task_download_moves(go2_task* t, uint8_t* moves) {
+  if (!t) return fail(GO2_E_ARG, "null task");
+  if (!t->launched) return fail(GO2_E_STATE, "task not launched");
+  if (t->count && !moves) return fail(GO2_E_ARG, "null moves");
+  if (t->count == 0) return GO2_OK;
+  hipError_t e = hipStreamSynchronize(t->stream);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+  if ((e = hipMemcpyAsync(moves, t->bufs.moves, t->count, hipMemcpyDeviceToHost, t->stream)) != hipSuccess)
+    return hip_fail(e, "hipMemcpyAsync(moves)");
   if ((e = hipStreamSynchronize(t->stream)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
   return GO2_OK;
 }

@@ -84,7 +84,9 @@ struct GraphCache {
 // Enqueue counter reset + solve kernel on `stream` (directly, or as a replay
 // of a captured graph when `graph` is non-null); events bracket the launch
 // when non-null.
-int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_scores,
+// d_moves (may be null) receives each board's best move: a square 0..63,
+// GO2_MOVE_PASS or GO2_MOVE_NONE.
+int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_scores, uint8_t* d_moves,
                  const go2_config& cfg, Workspace& ws, hipStream_t stream, hipEvent_t ev0,
                  hipEvent_t ev1, GraphCache* graph = nullptr);
 

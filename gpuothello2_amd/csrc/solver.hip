@@ -121,9 +121,21 @@ __device__ __forceinline__ int child_type(int type, int done) {
 
 constexpr int kNoValue = -127;  // "no helper reported yet"
 
+// A child value on its way into a node, with the move that produced it:
+// ((value + 64) | exact) << 6 | square.  Values are even, so bit 0 of the
+// shifted field says whether the value is exact (it beat the alpha the child
+// was searched with); the largest key over a node's children names a move
+// that reaches the node's final value.  Negative = none.
+__device__ __forceinline__ int move_key(int v, bool exact, int sq) {
+  return (((v + 64) | (int)exact) << 6) | sq;
+}
+__device__ __forceinline__ int key_value(int k) { return ((k >> 6) & ~1) - 64; }
+constexpr int kMovePass = 64, kMoveNone = 65;
+
 struct SolveArgs {
   const go2_board* boards;
   int8_t* scores;
+  uint8_t* moves;  // best move per board (square, kMovePass, kMoveNone), or null
   unsigned int n;
   int depth;          // DEPTH: boards with more empties are rejected
   int kff;            // fastest-first ordering at nodes with >= kff empties
@@ -231,7 +243,8 @@ struct HelpSlot {
   int alpha, beta, ctype, owner_E;
   unsigned int owner;  // global lane id of the owner
   unsigned int state;  // kSlotEmpty / kSlotWriting / full (tag << 2 | kSlotFull) / kSlotTaken
-  unsigned int pad[8];
+  int sq;              // the sibling's move square
+  unsigned int pad[7];
 };
 constexpr unsigned int kSlotEmpty = 0, kSlotWriting = 1, kSlotFull = 2, kSlotTaken = 3;
 // A full slot carries its owner's tag (global lane id < 2^19, frame level
@@ -368,8 +381,10 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
   int helpers_out = 0;          // helpers still working on this lane's frames
   int dead_level = 64;          // frame whose helpers must stop: E, 0 = all, 64 = none
   bool dead_new = false;        // dead_level changed; helpers not told yet
-  int report_value = 0;
+  int report_key = 0;           // move_key of a finished task, from the owner's side
   bool report_aborted = false;
+  int task_tag = 0;             // (owner alpha + 64) << 6 | move square of a helper's task
+  int root_key = -1;            // best move_key so far at the root of this lane's own board
   bool queue_done = false;      // wavefront-uniform
   unsigned long long c_pos = 0, c_nodes = 0, c_steps = 0, c_iters = 0, c_steals = 0, c_slots = 0;
   unsigned long long c_probes = 0, c_hits = 0, c_cuts = 0, c_stores = 0, c_exports = 0;
@@ -399,8 +414,9 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
   // This lane starts searching child (cP, cO) of frame oe of its owner, with
   // the owner's window (ca, cb) on that frame.
   auto start_child_task = [&](uint64_t cP, uint64_t cO, int ca, int cb, int ct, int own, int oe,
-                              bool ext) {
+                              bool ext, int csq) {
     ++c_nodes;
+    task_tag = ((ca + 64) << 6) | csq;
     E = oe - 1;
     Eroot = E;
     owner = own;
@@ -427,7 +443,8 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
         enter_node(om, ct);
         mode = SEARCH;
       } else {
-        report_value = -final_score(cP, cO);
+        const int rv = -final_score(cP, cO);
+        report_key = move_key(rv, rv > ca, csq);
         report_aborted = false;
         mode = REPORT;
       }
@@ -472,8 +489,13 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
             if ((P & O) || E > a.depth) {
               atomicCAS(&a.ctr->error, 0u, i + 1);
               a.scores[i] = 0;
+              if (a.moves) a.moves[i] = (uint8_t)kMoveNone;
             } else if (E <= 1) {
-              a.scores[i] = (int8_t)(E == 0 ? final_score(P, O) : last_score(P, O, ctz64(~(P | O))));
+              const int sq = ctz64(~(P | O));
+              a.scores[i] = (int8_t)(E == 0 ? final_score(P, O) : last_score(P, O, sq));
+              if (a.moves)
+                a.moves[i] = (uint8_t)(E == 0 ? kMoveNone
+                                              : flip(P, O, sq) ? sq : flip(O, P, sq) ? kMovePass : kMoveNone);
             } else {
               uint64_t m = mobility(P, O);
               negate = false;
@@ -482,6 +504,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
                 const uint64_t om = mobility(O, P);
                 if (!om) {
                   a.scores[i] = (int8_t)final_score(P, O);
+                  if (a.moves) a.moves[i] = (uint8_t)kMoveNone;
                   done = true;
                 } else {
                   const uint64_t t = P;
@@ -496,6 +519,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
                 alpha = -64;
                 beta = 64;
                 owner = -1;
+                root_key = -1;
                 enter_node(m, kPV);
                 mode = SEARCH;
               }
@@ -534,11 +558,11 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
           HelpSlot* hs = &a.slots[sidx];
           const uint64_t cP = ld_agent(&hs->P), cO = ld_agent(&hs->O);
           const int ca = ld_agent(&hs->alpha), cb = ld_agent(&hs->beta);
-          const int ct = ld_agent(&hs->ctype), oe = ld_agent(&hs->owner_E);
+          const int ct = ld_agent(&hs->ctype), oe = ld_agent(&hs->owner_E), csq = ld_agent(&hs->sq);
           const unsigned int og = ld_agent(&hs->owner);
           __hip_atomic_store(&hs->state, kSlotEmpty, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
           ++c_steals;
-          start_child_task(cP, cO, ca, cb, ct, (int)og, oe, true);
+          start_child_task(cP, cO, ca, cb, ct, (int)og, oe, true, csq);
         }
       } else {
         unsigned int act = 0;
@@ -555,7 +579,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
     // ---- 2. helpers report to their owners, one at a time ----
     if (mode == REPORT && owner_ext) {
       const size_t m = mb_idx(owner_E, (unsigned int)owner);
-      if (!report_aborted) atomicMax(&a.mb_best[m], report_value);
+      if (!report_aborted) atomicMax(&a.mb_best[m], report_key);
       __hip_atomic_fetch_add(&a.mb_done[m], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
       mode = IDLE;
     }
@@ -576,7 +600,10 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
             sx -= got * (1u + kSxExtOne);
             helpers_out -= (int)got;
             const int mb = ld_agent(&a.mb_best[m]);
-            if (mb >= -64) sx = sx_with_best(sx, mb);
+            if (mb >= 0) {
+              sx = sx_with_best(sx, key_value(mb));
+              if (e == Eroot && owner < 0) root_key = max(root_key, mb);
+            }
           }
           // Waiting on this frame (or aborting): take back every sibling
           // nobody claimed.  Otherwise, after reports, drop the records of
@@ -633,11 +660,14 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
       rm &= rm - 1;
       const int o = __builtin_amdgcn_readlane(owner, r);
       const int oe = __builtin_amdgcn_readlane(owner_E, r);
-      const int val = __builtin_amdgcn_readlane(report_value, r);
+      const int key = __builtin_amdgcn_readlane(report_key, r);
       const int ab = __builtin_amdgcn_readlane((int)report_aborted, r);
       if (lane == o) {
         uint32_t sx = get_sx(oe) - 1u;
-        if (!ab) sx = sx_with_best(sx, val);
+        if (!ab) {
+          sx = sx_with_best(sx, key_value(key));
+          if (oe == Eroot && owner < 0) root_key = max(root_key, key);
+        }
         set_sx(oe, sx);
         --helpers_out;
       }
@@ -677,7 +707,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
       const int e = 31 - __builtin_clz(lv) + 3;
       const int victim = __builtin_ctzll(__ballot((split_mask >> (e - 3)) & 1));
       const int thief = __builtin_ctzll(idle);
-      int give = 0, ca = 0, cb = 0, ct = 0;
+      int give = 0, ca = 0, cb = 0, ct = 0, csq = 0;
       uint64_t cP = 0, cO = 0;
       if (lane == victim) {
         uint64_t mv = get_mv(e), od = get_od(e);
@@ -710,6 +740,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
           ca = am;
           cb = fbeta;
           ct = child_type((int)(w >> 30), 1);
+          csq = sq;
           give = 1;
         }
       }
@@ -720,9 +751,10 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
         ca = __builtin_amdgcn_readlane(ca, victim);
         cb = __builtin_amdgcn_readlane(cb, victim);
         ct = __builtin_amdgcn_readlane(ct, victim);
+        csq = __builtin_amdgcn_readlane(csq, victim);
         if (lane == thief) {
           ++c_steals;
-          start_child_task(cP, cO, ca, cb, ct, victim, e, false);
+          start_child_task(cP, cO, ca, cb, ct, victim, e, false, csq);
         }
       }
     } else if (a.slots && demand && !idle && __ballot((split_mask >> (dem_e - 3)) != 0)) {
@@ -794,6 +826,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
             st_agent(&hs->ctype, child_type((int)(w >> 30), 1));
             st_agent(&hs->owner_E, e);
             st_agent(&hs->owner, gtid);
+            st_agent(&hs->sq, sq);
             __hip_atomic_store(&hs->state, slot_full(gtid, e, sq), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             ++c_exports;
           }
@@ -939,6 +972,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
         enter_node(cm, ct);
       }
       if (commit) {
+        if (E == Eroot && owner < 0) root_key = max(root_key, move_key(v, v > alpha, sq));
         if (v > best) bestmv = sq;
         best = max(best, v);
         alpha = max(alpha, best);
@@ -1000,9 +1034,11 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
           const int r = negate ? -best : best;
           if (owner < 0) {
             a.scores[idx] = (int8_t)r;
+            if (a.moves)
+              a.moves[idx] = (uint8_t)(negate ? kMovePass : root_key >= 0 ? (root_key & 63) : kMoveNone);
             mode = IDLE;
           } else {
-            report_value = r;
+            report_key = move_key(r, r > (task_tag >> 6) - 64, task_tag & 63);
             report_aborted = false;
             mode = REPORT;
           }
@@ -1034,6 +1070,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
         const int pbest = frame_val(w, 14);
         undo_move(P, O, fb, w);
         const int vv = ((w >> 27) & 1) ? best : -best;
+        if (E == Eroot && owner < 0) root_key = max(root_key, move_key(vv, vv > palpha, (int)((w >> 21) & 63)));
         beta = pbeta;
         best = max(pbest, vv);
         alpha = max(palpha, best);
@@ -1233,7 +1270,7 @@ int Workspace::ensure(size_t frame_bytes) {
   return GO2_OK;
 }
 
-int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_scores,
+int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_scores, uint8_t* d_moves,
                  const go2_config& cfg, Workspace& ws, hipStream_t stream, hipEvent_t ev0,
                  hipEvent_t ev1, GraphCache* graph) {
   if (n > 0xFFFFFF00ULL) return fail(GO2_E_ARG, "too many boards in one solve (max 2^32-256)");
@@ -1294,6 +1331,7 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   SolveArgs args{};
   args.boards = d_boards;
   args.scores = d_scores;
+  args.moves = d_moves;
   args.n = (unsigned int)n;
   args.depth = max_depth;
   args.kff = cfg.mobility_min_empties;
@@ -1361,7 +1399,7 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   if (off > ws.frames_bytes) {
     rc = ws.ensure(off);
     if (rc != GO2_OK) return rc;
-    return launch_solve(d_boards, n, max_depth, d_scores, cfg, ws, stream, ev0, ev1, graph);
+    return launch_solve(d_boards, n, max_depth, d_scores, d_moves, cfg, ws, stream, ev0, ev1, graph);
   }
   // s_memrealtime runs at 100 MHz; each wavefront adds its own start time.
   if (cfg.time_limit_ms > 0) args.limit_ticks = (unsigned long long)cfg.time_limit_ms * 100000ULL;

@@ -35,6 +35,11 @@ extern "C" {
 /* Largest number of empty squares one board may have. */
 #define GO2_MAX_EMPTIES 32
 
+/* Best-move codes besides squares 0..63 (bit 0 = A1), the reference's
+ * hand::PASS / hand::NOMOVE (alphabeta.cuh:8-9). */
+#define GO2_MOVE_PASS 64 /* the side to move has no legal move; the opponent has */
+#define GO2_MOVE_NONE 65 /* game over (or an invalid board) */
+
 /* Move ordering of the solver. */
 #define GO2_ORDER_STATIC 0   /* corners, then plain squares, X-squares last */
 #define GO2_ORDER_MOBILITY 1 /* fastest-first (fewest replies) above a cutoff, parity below */
@@ -130,6 +135,17 @@ int go2_solve(const go2_board *boards, size_t n, int max_depth, int8_t *scores,
 int go2_solve_device(const go2_board *d_boards, size_t n, int max_depth, int8_t *d_scores,
                      const go2_config *cfg, go2_stats *stats, void *hip_stream);
 
+/* As go2_solve / go2_solve_device, plus one best move per board in `moves`:
+ * a square whose child has the board's exact score (any one of them when
+ * several do), GO2_MOVE_PASS or GO2_MOVE_NONE.  The reference returns a best
+ * move only from think mode (thinker.cu:169, printed by main.cu:114-117);
+ * this is the solve path's counterpart. */
+int go2_solve_moves(const go2_board *boards, size_t n, int max_depth, int8_t *scores,
+                    uint8_t *moves, const go2_config *cfg, go2_stats *stats);
+int go2_solve_device_moves(const go2_board *d_boards, size_t n, int max_depth, int8_t *d_scores,
+                           uint8_t *d_moves, const go2_config *cfg, go2_stats *stats,
+                           void *hip_stream);
+
 /* ---- batched task (replaces BatchedTask, reference solver.cuh:8-25,
  *      solver.cu:313-348) ------------------------------------------------------------ */
 typedef struct go2_task go2_task;
@@ -149,6 +165,9 @@ int go2_task_wait(go2_task *task);
 /* Copy scores (count entries) and stats out (the reference's result[] and
  * *total, main.cu:165-171).  Waits for the task first. */
 int go2_task_download(go2_task *task, int8_t *scores, go2_stats *stats);
+/* Copy the best moves (count entries, codes as go2_solve_moves) of the last
+ * launch out.  Waits for the task first. */
+int go2_task_download_moves(go2_task *task, uint8_t *moves);
 /* Free device buffers and the stream (reference solver.cu:339-348). */
 void go2_task_destroy(go2_task *task);
 

@@ -325,6 +325,75 @@ double orc_solve_batch(const uint64_t *P, const uint64_t *O, size_t n, int depth
   return (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
 }
 
+/* Optimal moves: every legal square whose child solves to the position's
+ * exact score (the definition of an optimal move; any one of them is a valid
+ * best move for a solver to return).  Returns the score. */
+int orc_best_moves(uint64_t P, uint64_t O, int depth, uint64_t *optimal, int *kind) {
+  const uint64_t m = orc_mobility(P, O);
+  *optimal = 0;
+  if (!m) {
+    if (orc_mobility(O, P)) {
+      *kind = ORC_MOVES_PASS;
+      return -orc_solve_ref(O, P, depth, NULL);
+    }
+    *kind = ORC_MOVES_GAME_OVER;
+    return orc_final_score(P, O);
+  }
+  *kind = ORC_MOVES_PLAY;
+  int vals[64];
+  int best = -65;
+  for (uint64_t r = m; r; r &= r - 1) {
+    const int sq = __builtin_ctzll(r);
+    const uint64_t bit = 1ULL << sq;
+    const uint64_t f = orc_flip(P, O, sq);
+    vals[sq] = -orc_solve_ref(O ^ f, (P ^ f) | bit, depth > 0 ? depth - 1 : 0, NULL);
+    if (vals[sq] > best) best = vals[sq];
+  }
+  for (uint64_t r = m; r; r &= r - 1)
+    if (vals[__builtin_ctzll(r)] == best) *optimal |= r & (0 - r);
+  return best;
+}
+
+typedef struct {
+  const uint64_t *P, *O;
+  size_t n;
+  int depth;
+  int8_t *scores;
+  uint64_t *optimal;
+  uint8_t *kinds;
+  atomic_size_t next;
+} moves_job;
+
+static void *moves_worker(void *arg) {
+  moves_job *j = (moves_job *)arg;
+  for (;;) {
+    size_t i = atomic_fetch_add(&j->next, 1);
+    if (i >= j->n) break;
+    int kind = 0;
+    j->scores[i] = (int8_t)orc_best_moves(j->P[i], j->O[i], j->depth, &j->optimal[i], &kind);
+    j->kinds[i] = (uint8_t)kind;
+  }
+  return NULL;
+}
+
+void orc_best_moves_batch(const uint64_t *P, const uint64_t *O, size_t n, int depth, int threads,
+                          int8_t *scores, uint64_t *optimal, uint8_t *kinds) {
+  if (threads < 1) threads = 1;
+  if (threads > 256) threads = 256;
+  moves_job j;
+  j.P = P;
+  j.O = O;
+  j.n = n;
+  j.depth = depth;
+  j.scores = scores;
+  j.optimal = optimal;
+  j.kinds = kinds;
+  atomic_init(&j.next, 0);
+  pthread_t tid[256];
+  for (int t = 0; t < threads; ++t) pthread_create(&tid[t], NULL, moves_worker, &j);
+  for (int t = 0; t < threads; ++t) pthread_join(tid[t], NULL);
+}
+
 /* ------------------------------------------------------------------------- */
 /* Synthetic positions: random legal moves from the standard start (black to
  * move: D5,E4 black; D4,E5 white), passing when forced, until exactly E

@@ -51,6 +51,16 @@ int orc_minimax(uint64_t P, uint64_t O, uint64_t *nodes);
 double orc_solve_batch(const uint64_t *P, const uint64_t *O, size_t n, int depth,
                        int threads, int8_t *scores, uint64_t *nodes);
 
+/* Optimal moves of (P, O): *optimal = the legal squares whose child solves
+ * to the exact score; *kind = ORC_MOVES_PLAY, or ORC_MOVES_PASS (P has no
+ * move, O has) / ORC_MOVES_GAME_OVER with *optimal = 0.  Returns the score. */
+#define ORC_MOVES_PLAY 0
+#define ORC_MOVES_PASS 1
+#define ORC_MOVES_GAME_OVER 2
+int  orc_best_moves(uint64_t P, uint64_t O, int depth, uint64_t *optimal, int *kind);
+void orc_best_moves_batch(const uint64_t *P, const uint64_t *O, size_t n, int depth, int threads,
+                          int8_t *scores, uint64_t *optimal, uint8_t *kinds);
+
 /* ---- synthetic positions ---------------------------------------------------- */
 uint64_t orc_splitmix64(uint64_t *state);
 uint64_t orc_position_key(uint64_t seed, uint32_t config, uint64_t index, uint32_t attempt);

@@ -42,6 +42,10 @@ def oracle() -> ctypes.CDLL:
         L.orc_solve_ref.argtypes = [u64, u64, i32, P64]
         L.orc_minimax.argtypes = [u64, u64, P64]
         L.orc_solve_batch.argtypes = [ctypes.c_void_p, ctypes.c_void_p, sz, i32, i32, ctypes.c_void_p, P64]
+        L.orc_best_moves.argtypes = [u64, u64, i32, P64, ctypes.POINTER(ctypes.c_int)]
+        L.orc_best_moves_batch.argtypes = [ctypes.c_void_p, ctypes.c_void_p, sz, i32, i32, ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_best_moves_batch.restype = None
         L.orc_solve_batch.restype = ctypes.c_double
         L.orc_position_key.argtypes = [u64, u32, u64, u32]
         L.orc_position_key.restype = u64
@@ -125,6 +129,31 @@ def solve_batch(boards: np.ndarray, depth: int, threads: int = 8):
     secs = oracle().orc_solve_batch(P.ctypes.data, O.ctypes.data, b.shape[0], depth, threads,
                                     scores.ctypes.data, ctypes.byref(nodes))
     return scores, nodes.value, secs
+
+
+MOVES_PLAY, MOVES_PASS, MOVES_GAME_OVER = 0, 1, 2
+
+
+def best_moves(p: int, o: int, depth: int = 60):
+    """(score, mask of optimal squares, kind) of one board."""
+    m = u64(0)
+    k = ctypes.c_int(0)
+    s = oracle().orc_best_moves(p, o, depth, ctypes.byref(m), ctypes.byref(k))
+    return s, m.value, k.value
+
+
+def best_moves_batch(boards: np.ndarray, depth: int, threads: int = 8):
+    """(scores, optimal-square masks, kinds) of (n,2) boards, multithreaded."""
+    b = np.ascontiguousarray(boards, dtype=np.uint64).reshape(-1, 2)
+    P = np.ascontiguousarray(b[:, 0])
+    O = np.ascontiguousarray(b[:, 1])
+    n = b.shape[0]
+    scores = np.zeros(n, dtype=np.int8)
+    masks = np.zeros(n, dtype=np.uint64)
+    kinds = np.zeros(n, dtype=np.uint8)
+    oracle().orc_best_moves_batch(P.ctypes.data, O.ctypes.data, n, depth, threads, scores.ctypes.data,
+                                  masks.ctypes.data, kinds.ctypes.data)
+    return scores, masks, kinds
 
 
 def minimax(p: int, o: int) -> int:

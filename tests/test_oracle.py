@@ -186,3 +186,41 @@ def test_planted_mistake_is_caught(O):
             best = max(best, ol.minimax(o ^ f, (p ^ f) | bit))  # sign dropped on purpose
         wrong += best != int(s)
     assert wrong > 0
+
+
+def test_best_moves_equal_brute_force(O):
+    # Optimal squares by definition: legal moves whose child's no-pruning
+    # minimax value, negated, equals the position's value.
+    d = np.load(os.path.join(GOLD, "solve_small.npz"))
+    checked = {ol.MOVES_PLAY: 0, ol.MOVES_PASS: 0, ol.MOVES_GAME_OVER: 0}
+    keep = d["empties"] <= 8
+    for p, o, s in zip(d["player"][keep], d["opponent"][keep], d["score"][keep]):
+        p, o = int(p), int(o)
+        score, mask, kind = ol.best_moves(p, o)
+        assert score == int(s)
+        m = O.orc_mobility(p, o)
+        if m:
+            assert kind == ol.MOVES_PLAY
+            want = 0
+            while m:
+                bit = m & -m
+                m ^= bit
+                f = O.orc_flip(p, o, bit.bit_length() - 1)
+                if -ol.minimax(o ^ f, (p ^ f) | bit) == score:
+                    want |= bit
+            assert mask == want and want != 0
+        else:
+            assert mask == 0
+            assert kind == (ol.MOVES_PASS if O.orc_mobility(o, p) else ol.MOVES_GAME_OVER)
+        checked[kind] += 1
+    assert checked[ol.MOVES_PLAY] > 100 and checked[ol.MOVES_PASS] > 0 and checked[ol.MOVES_GAME_OVER] > 0
+
+
+def test_best_moves_batch_matches_single():
+    b = ol.generate(31, 7, 40, 11)
+    sc, masks, kinds = ol.best_moves_batch(b, 11, threads=4)
+    ref, _, _ = ol.solve_batch(b, 11)
+    assert np.array_equal(sc, ref)
+    for i in range(len(b)):
+        s, m, k = ol.best_moves(int(b[i, 0]), int(b[i, 1]), 11)
+        assert (s, m, k) == (int(sc[i]), int(masks[i]), int(kinds[i]))
