@@ -192,4 +192,53 @@ GO2_HD int last_score_tab(uint64_t P, uint64_t O, int sq, const T& t) {
   return final_score(P, O);
 }
 
+// Both moves of O after P, with two empty squares a < b, has passed (rare;
+// kept out of line).  Returns the score for P, 65 in *best_sq when the game
+// is over.
+template <class T>
+GO2_HD int solve2_pass(uint64_t P, uint64_t O, int a, int b, const T& t,
+                                                  int& nodes, int* best_sq) {
+  const uint64_t ba = 1ULL << a, bb = 1ULL << b;
+  const uint64_t ga = flip(O, P, a), gb = flip(O, P, b);
+  if (!(ga | gb)) {
+    if (best_sq) *best_sq = 65;
+    return final_score(P, O);
+  }
+  const int wa = ga ? -last_score_tab(P ^ ga, O ^ ga ^ ba, b, t) : -65;
+  const int wb = gb ? -last_score_tab(P ^ gb, O ^ gb ^ bb, a, t) : -65;
+  nodes += 1 + (ga != 0) + (gb != 0);
+  if (best_sq) *best_sq = 64;
+  return -(wa > wb ? wa : wb);
+}
+
+// Exact score for P to move with exactly two empty squares, in straight-line
+// code: each move of P leaves one empty square, scored directly; if P has no
+// move, O's moves (or the final count) decide.  `nodes` counts the children
+// evaluated.  `best_sq` (when non-null) receives the move that reaches the
+// score: a square, 64 for a pass, 65 when the game is over.
+template <class T>
+GO2_HD int solve2(uint64_t P, uint64_t O, const T& t, int& nodes, int* best_sq = nullptr) {
+  const uint64_t e = ~(P | O);
+  const int a = ctz64(e);
+  const int b = 63 - __builtin_clzll(e & (e - 1));
+  int best = -65;
+  uint64_t f = flip(P, O, a);
+  if (f) {
+    best = -last_score_tab(O ^ f, P ^ f ^ (1ULL << a), b, t);
+    ++nodes;
+    if (best_sq) *best_sq = a;
+  }
+  f = flip(P, O, b);
+  if (f) {
+    const int v = -last_score_tab(O ^ f, P ^ f ^ (1ULL << b), a, t);
+    ++nodes;
+    if (v > best) {
+      best = v;
+      if (best_sq) *best_sq = b;
+    }
+  }
+  if (best > -65) return best;
+  return solve2_pass(P, O, a, b, t, nodes, best_sq);
+}
+
 }  // namespace go2

@@ -26,7 +26,7 @@
 //     while helpers still search its siblings stops them (and their own
 //     helpers, one level per loop trip) instead of waiting them out.
 //   * The node being searched lives in VGPRs.  Ancestors are spilled to a
-//     stack, one frame per empties level: levels 3..(lds_levels+2) in LDS
+//     stack, one frame per empties level: levels 4..(lds_levels+3) in LDS
 //     (struct-of-arrays, lane index fastest, bank-conflict free), higher
 //     levels in HBM (same layout over all lanes of the grid; touched only
 //     by the rare nodes near the root).  A frame holds the flip set of the
@@ -35,8 +35,9 @@
 //     Nodes at split levels also keep their board, a pending-helper count
 //     and the best value helpers reported, in HBM.
 //   * Every loop iteration does the same heavy work on every searching
-//     lane: one flip and one mobility (or, one ply from the end, the exact
-//     score of the last empty square).  Move ordering work is spread over
+//     lane: one flip and one mobility (or, at nodes with three empties, the
+//     exact value of the two-empties child in straight-line code, so nodes
+//     with two empties never become frames).  Move ordering work is spread over
 //     iterations ("scoring" steps: one child per iteration).
 //   * Exact fail-soft negamax, full window (-64, 64) at the root; passes
 //     are folded into the child frame (board swapped, window kept, value not
@@ -120,6 +121,9 @@ __device__ __forceinline__ int child_type(int type, int done) {
 }
 
 constexpr int kNoValue = -127;  // "no helper reported yet"
+// Lowest stack level: nodes with three empties solve their children in
+// straight-line code, so only nodes with four or more push frames.
+constexpr int kFrameBase = 4;
 
 // A child value on its way into a node, with the move that produced it:
 // ((value + 64) | exact) << 6 | square.  Values are even, so bit 0 of the
@@ -141,7 +145,7 @@ struct SolveArgs {
   int kff;            // fastest-first ordering at nodes with >= kff empties
   int split_min;      // helpers take siblings at nodes with >= split_min empties
   int cut_after;      // ... at expected CUT nodes only after this many children
-  int lds_levels;     // frame levels 3..lds_levels+2 live in LDS
+  int lds_levels;     // frame levels 4..lds_levels+3 live in LDS
   int K;              // boards owned by one wavefront at a time
   unsigned int total_lanes;
   unsigned long long limit_ticks;  // time limit in s_memrealtime ticks (100 MHz); 0 = none
@@ -176,7 +180,7 @@ struct SolveArgs {
   // of a sibling that may still be unclaimed, 0 = none
   unsigned int* mb_rec;
   SolveCounters* ctr;
-  // HBM frames of levels >= lds_levels+3: [level - lds_levels - 3][total_lanes]
+  // HBM frames of levels >= lds_levels+4: [level - lds_levels - 4][total_lanes]
   uint64_t* g_fb;
   uint64_t* g_mv;
   uint64_t* g_od;
@@ -316,9 +320,9 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
   // Every frame slot is read and written only by the lane that owns it.
   // LDS levels use ds_* accesses and HBM levels global ones (no flat
   // addressing on the hot path).
-  auto lds_k = [&](int E) { return (E - 3) * bs + tid; };
-  auto hbm_k = [&](int E) { return (size_t)(E - 3 - LL) * a.total_lanes + gtid; };
-  auto in_lds = [&](int E) { return E - 3 < LL; };
+  auto lds_k = [&](int E) { return (E - kFrameBase) * bs + tid; };
+  auto hbm_k = [&](int E) { return (size_t)(E - kFrameBase - LL) * a.total_lanes + gtid; };
+  auto in_lds = [&](int E) { return E - kFrameBase < LL; };
   auto get_sx = [&](int E) -> uint32_t { return in_lds(E) ? l_sx[lds_k(E)] : hbm_ld(&a.g_sx[hbm_k(E)]); };
   auto set_sx = [&](int E, uint32_t v) {
     if (in_lds(E)) {
@@ -374,20 +378,44 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
   bool demand = false;          // wavefront-uniform: some wavefront is idle
   int dem_e = 64;               // wavefront-uniform: lowest level to export from
   unsigned int poll_base = gtid - (unsigned int)lane;  // wavefront-uniform
-  int alpha0 = 0;               // current node's alpha at entry
-  int bestmv = 63;              // move that gave the current node's best value (63 = none)
+  // For the table's entries: the current node's alpha at entry, halved
+  // (bits 6-12, two's complement), and the move that gave its best value
+  // (bits 0-5, 63 = none), in one register.
+  int a0mv = 63;
+  auto alpha0_of = [&]() { return ((a0mv << 19) >> 25) * 2; };
+  auto bestmv_of = [&]() { return a0mv & 63; };
   unsigned int idx = 0;
   uint32_t split_mask = 0;      // bit E-3: this lane's frame at level E offers siblings
   int helpers_out = 0;          // helpers still working on this lane's frames
   int dead_level = 64;          // frame whose helpers must stop: E, 0 = all, 64 = none
   bool dead_new = false;        // dead_level changed; helpers not told yet
-  int report_key = 0;           // move_key of a finished task, from the owner's side
   bool report_aborted = false;
-  int task_tag = 0;             // (owner alpha + 64) << 6 | move square of a helper's task
-  int root_key = -1;            // best move_key so far at the root of this lane's own board
+  // One register, three uses that never overlap: a board's owner keeps the
+  // best move_key so far at the board's root (-1 = none); a helper keeps
+  // (owner alpha + 64) << 6 | move square of its task while searching it,
+  // then the task's move_key while reporting it.
+  int mkey = -1;
   bool queue_done = false;      // wavefront-uniform
-  unsigned long long c_pos = 0, c_nodes = 0, c_steps = 0, c_iters = 0, c_steals = 0, c_slots = 0;
-  unsigned long long c_probes = 0, c_hits = 0, c_cuts = 0, c_stores = 0, c_exports = 0;
+  // Per-lane statistics, 32-bit (a lane adds a few per trip) and flushed to
+  // the 64-bit launch counters every 65,536 trips and at exit.
+  unsigned int c_pos = 0, c_nodes = 0, c_steps = 0, c_iters = 0, c_steals = 0;
+  unsigned int c_probes = 0, c_hits = 0, c_cuts = 0, c_stores = 0, c_exports = 0;
+  auto flush_counters = [&]() {
+    auto add = [](unsigned long long* dst, unsigned int& v) {
+      if (v) atomicAdd(dst, (unsigned long long)v);
+      v = 0;
+    };
+    add(&a.ctr->positions, c_pos);
+    add(&a.ctr->nodes, c_nodes);
+    add(&a.ctr->score_steps, c_steps);
+    add(&a.ctr->iterations, c_iters);
+    add(&a.ctr->steals, c_steals);
+    add(&a.ctr->exports, c_exports);
+    add(&a.ctr->tt_probes, c_probes);
+    add(&a.ctr->tt_hits, c_hits);
+    add(&a.ctr->tt_cuts, c_cuts);
+    add(&a.ctr->tt_stores, c_stores);
+  };
   auto mb_idx = [&](int e, unsigned int g) { return (size_t)(e - a.ext_lo) * a.total_lanes + g; };
   auto rec_idx = [&](int e, int k, unsigned int g) {
     return ((size_t)(e - a.ext_lo) * kExtRec + (size_t)k) * a.total_lanes + g;
@@ -406,8 +434,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
     ntype = type;
     pchk = false;
     pop_pending = false;
-    alpha0 = alpha;
-    bestmv = 63;
+    a0mv = (((alpha >> 1) & 127) << 6) | 63;
     if (E >= a.split_min) set_sx(E, sx_init(alpha));
   };
 
@@ -416,7 +443,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
   auto start_child_task = [&](uint64_t cP, uint64_t cO, int ca, int cb, int ct, int own, int oe,
                               bool ext, int csq) {
     ++c_nodes;
-    task_tag = ((ca + 64) << 6) | csq;
+    mkey = ((ca + 64) << 6) | csq;
     E = oe - 1;
     Eroot = E;
     owner = own;
@@ -444,7 +471,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
         mode = SEARCH;
       } else {
         const int rv = -final_score(cP, cO);
-        report_key = move_key(rv, rv > ca, csq);
+        mkey = move_key(rv, rv > ca, csq);
         report_aborted = false;
         mode = REPORT;
       }
@@ -458,6 +485,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
   for (;;) {
     // ---- time limit (checked every 256 trips; uniform per wavefront) ----
     ++trip;
+    if ((trip & 0xFFFF) == 0) flush_counters();
     if (a.limit_ticks && (trip & 255) == 0) {
       if (__builtin_amdgcn_s_memrealtime() > deadline) {
         if (lane == 0) atomicExch(&a.ctr->error, kTimedOut);
@@ -496,6 +524,11 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
               if (a.moves)
                 a.moves[i] = (uint8_t)(E == 0 ? kMoveNone
                                               : flip(P, O, sq) ? sq : flip(O, P, sq) ? kMovePass : kMoveNone);
+            } else if (E == 2) {
+              int nl = 0, mv = kMoveNone;
+              a.scores[i] = (int8_t)solve2(P, O, lastc, nl, &mv);
+              c_nodes += nl;
+              if (a.moves) a.moves[i] = (uint8_t)mv;
             } else {
               uint64_t m = mobility(P, O);
               negate = false;
@@ -519,7 +552,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
                 alpha = -64;
                 beta = 64;
                 owner = -1;
-                root_key = -1;
+                mkey = -1;
                 enter_node(m, kPV);
                 mode = SEARCH;
               }
@@ -579,7 +612,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
     // ---- 2. helpers report to their owners, one at a time ----
     if (mode == REPORT && owner_ext) {
       const size_t m = mb_idx(owner_E, (unsigned int)owner);
-      if (!report_aborted) atomicMax(&a.mb_best[m], report_key);
+      if (!report_aborted) atomicMax(&a.mb_best[m], mkey);
       __hip_atomic_fetch_add(&a.mb_done[m], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
       mode = IDLE;
     }
@@ -602,7 +635,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
             const int mb = ld_agent(&a.mb_best[m]);
             if (mb >= 0) {
               sx = sx_with_best(sx, key_value(mb));
-              if (e == Eroot && owner < 0) root_key = max(root_key, mb);
+              if (e == Eroot && owner < 0) mkey = max(mkey, mb);
             }
           }
           // Waiting on this frame (or aborting): take back every sibling
@@ -660,13 +693,13 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
       rm &= rm - 1;
       const int o = __builtin_amdgcn_readlane(owner, r);
       const int oe = __builtin_amdgcn_readlane(owner_E, r);
-      const int key = __builtin_amdgcn_readlane(report_key, r);
+      const int key = __builtin_amdgcn_readlane(mkey, r);
       const int ab = __builtin_amdgcn_readlane((int)report_aborted, r);
       if (lane == o) {
         uint32_t sx = get_sx(oe) - 1u;
         if (!ab) {
           sx = sx_with_best(sx, key_value(key));
-          if (oe == Eroot && owner < 0) root_key = max(root_key, key);
+          if (oe == Eroot && owner < 0) mkey = max(mkey, key);
         }
         set_sx(oe, sx);
         --helpers_out;
@@ -839,7 +872,6 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
     // whose side has no move is settled next trip with the other side's
     // mobility in the same slot (pass or game over), so no trip pays for a
     // second mobility.
-    c_slots += 1;
     bool commit = false;
     int v = 0;
     if (mode == SEARCH && !pop_pending) {
@@ -860,7 +892,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
           sq = pick_static<ORDER>(moves, ~(P | O));
         }
         f = flip(P, O, sq);
-        leaf = !scoring && E == 2;
+        leaf = !scoring && E == 3;
       }
       const uint64_t bit = 1ULL << sq;
       const uint64_t cP = O ^ f, cO = P ^ f ^ bit;
@@ -868,7 +900,10 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
       my = pchk ? cP : cO;
       uint64_t cm = 0;
       if (leaf) {
-        v = -last_score_tab(cP, cO, ctz64(~(cP | cO)), lastc);
+        // the child has two empty squares: its exact value in this trip
+        int nl = 0;
+        v = -solve2(cP, cO, lastc, nl);
+        c_nodes += nl;
       } else {
         cm = mobility(mx, my);
       }
@@ -958,7 +993,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
         const int thr = ntype == kCut ? (tail ? 1 : a.cut_after) : (tail && ntype == kAll ? 0 : 1);
         if (E >= a.split_min && moves && ndone >= thr) split_mask |= 1u << (E - 3);
         const int ct = child_type(ntype, ndone);
-        if (a.tt && E >= a.tt_min) set_sx(E, (get_sx(E) & ~(63u << 23)) | ((uint32_t)bestmv << 23));
+        if (a.tt && E >= a.tt_min) set_sx(E, (get_sx(E) & ~(63u << 23)) | ((uint32_t)bestmv_of() << 23));
         if (passed) {
           P = cO;
           O = cP;
@@ -972,8 +1007,8 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
         enter_node(cm, ct);
       }
       if (commit) {
-        if (E == Eroot && owner < 0) root_key = max(root_key, move_key(v, v > alpha, sq));
-        if (v > best) bestmv = sq;
+        if (E == Eroot && owner < 0) mkey = max(mkey, move_key(v, v > alpha, sq));
+        if (v > best) a0mv = (a0mv & ~63) | sq;
         best = max(best, v);
         alpha = max(alpha, best);
         ndone = min(ndone + 1, 3);
@@ -1024,8 +1059,8 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
           // Store the finished node: fail low -> upper bound, fail high ->
           // lower bound, inside the window -> exact.
           ++c_stores;
-          const int lo = best > alpha0 ? best : -64, up = best < beta ? best : 64;
-          const uint64_t d = (uint64_t)(lo + 64) | ((uint64_t)(up + 64) << 8) | ((uint64_t)bestmv << 16) |
+          const int lo = best > alpha0_of() ? best : -64, up = best < beta ? best : 64;
+          const uint64_t d = (uint64_t)(lo + 64) | ((uint64_t)(up + 64) << 8) | ((uint64_t)bestmv_of() << 16) |
                              ((uint64_t)tt_gen << 24) | ((uint64_t)E << 32);
           const uint64_t h = zobrist(P, O, a.zob);
           a.tt[h & a.tt_mask] = make_ulonglong2(h ^ d, d);
@@ -1035,10 +1070,10 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
           if (owner < 0) {
             a.scores[idx] = (int8_t)r;
             if (a.moves)
-              a.moves[idx] = (uint8_t)(negate ? kMovePass : root_key >= 0 ? (root_key & 63) : kMoveNone);
+              a.moves[idx] = (uint8_t)(negate ? kMovePass : mkey >= 0 ? (mkey & 63) : kMoveNone);
             mode = IDLE;
           } else {
-            report_key = move_key(r, r > (task_tag >> 6) - 64, task_tag & 63);
+            mkey = move_key(r, r > (mkey >> 6) - 64, mkey & 63);
             report_aborted = false;
             mode = REPORT;
           }
@@ -1070,14 +1105,13 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
         const int pbest = frame_val(w, 14);
         undo_move(P, O, fb, w);
         const int vv = ((w >> 27) & 1) ? best : -best;
-        if (E == Eroot && owner < 0) root_key = max(root_key, move_key(vv, vv > palpha, (int)((w >> 21) & 63)));
+        if (E == Eroot && owner < 0) mkey = max(mkey, move_key(vv, vv > palpha, (int)((w >> 21) & 63)));
         beta = pbeta;
         best = max(pbest, vv);
         alpha = max(palpha, best);
         if (E >= a.split_min) {
           const uint32_t psx = get_sx(E);
-          alpha0 = sx_alpha0(psx);
-          bestmv = vv > pbest ? (int)((w >> 21) & 63) : (int)((psx >> 23) & 63);
+          a0mv = (((sx_alpha0(psx) >> 1) & 127) << 6) | (vv > pbest ? (int)((w >> 21) & 63) : (int)((psx >> 23) & 63));
         }
         ndone = min((int)((w >> 28) & 3) + 1, 3);
         ntype = (int)(w >> 30);
@@ -1086,19 +1120,8 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
     }
   }
 
-  atomicAdd(&a.ctr->positions, c_pos);
-  atomicAdd(&a.ctr->nodes, c_nodes);
-  atomicAdd(&a.ctr->score_steps, c_steps);
-  atomicAdd(&a.ctr->iterations, c_iters);
-  atomicAdd(&a.ctr->steals, c_steals);
-  atomicAdd(&a.ctr->lane_slots, c_slots);
-  if (a.slots) atomicAdd(&a.ctr->exports, c_exports);
-  if (a.tt) {
-    atomicAdd(&a.ctr->tt_probes, c_probes);
-    atomicAdd(&a.ctr->tt_hits, c_hits);
-    atomicAdd(&a.ctr->tt_cuts, c_cuts);
-    atomicAdd(&a.ctr->tt_stores, c_stores);
-  }
+  flush_counters();
+  if (lane == 0) atomicAdd(&a.ctr->lane_slots, 64ull * trip);  // every lane takes part in every trip
 }
 
 __global__ void reset_counters_kernel(SolveCounters* c) {
@@ -1277,8 +1300,8 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   KernelFn fn = nullptr;
   const int maxe = stack_levels_for(max_depth, &fn, cfg.ordering);
   const int bs = cfg.block_size;
-  const int levels = maxe - 2;  // frame levels 3..maxe
-  // Keep the stack in LDS up to 8 empties: three 256-lane blocks per CU (the
+  const int levels = maxe + 1 - kFrameBase;  // frame levels 4..maxe
+  // Keep the stack in LDS up to 9 empties: three 256-lane blocks per CU (the
   // kernel's register budget allows three wavefronts per SIMD) with the
   // last-square and base-3 tables; nodes nearer the root use HBM frames.
   int lds_levels = cfg.lds_levels > 0 ? cfg.lds_levels : 6;

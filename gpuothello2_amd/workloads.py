@@ -133,11 +133,11 @@ def c5(scale: float) -> dict:
     n = max(256, int(8192 * scale))
     learn.install()
     b = generate(505, 5, n, 20)
-    cfg = default_config(ordering=ORDER_LEARNED, tt_log2_buckets=26, tt_min_empties=10, time_limit_ms=0)
+    cfg = default_config(ordering=ORDER_LEARNED, tt_log2_buckets=26, tt_min_empties=10, time_limit_ms=900000)
     s, st, _ = _solve(b, 20, cfg, "c5")
     secs = st["kernel_ms"] / 1e3
     k = min(n, 256)
-    cfg2 = default_config(ordering=ORDER_LEARNED, time_limit_ms=0)
+    cfg2 = default_config(ordering=ORDER_LEARNED, time_limit_ms=300000)
     s2, st2, _ = _solve(b[:k], 20, cfg2, "c5 no table (prefix)")
     return {"config": "C5", "positions": n, "empties": 20, "table_buckets": 2 ** 26,
             "positions_per_s": n / secs, "nodes_per_position": st["nodes"] / n,
