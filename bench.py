@@ -4,10 +4,11 @@ One step = one exact solve of the whole batch of random-playout positions
 (default 100,000 at 16 empties), boards already resident in device memory
 (a go2 task uploaded once), timed from launch to completion.  The headline
 solver (learned move ordering when a trained table is present, else
-fastest-first mobility ordering) is compared with the plain batched kernel
-(static corner / X-square ordering) on the same GPU, and every score is
-checked: headline vs plain on the plain sample, headline vs the CPU oracle
-on the CPU sample.
+fastest-first mobility ordering, with a shared lock-free transposition table
+of 2^29 buckets probed at nodes with >= 8 empties, 16 boards per wavefront)
+is compared with the plain batched kernel (static corner / X-square
+ordering, no table) on the same GPU, and every score is checked: headline vs
+plain on the plain sample, headline vs the CPU oracle on the CPU sample.
 
 Prints ONE JSON line (see the keys below).  `--gpus N` with N > 1 exits:
 this solver runs on one GPU.
@@ -49,6 +50,10 @@ def main() -> int:
     ap.add_argument("--empties", type=int, default=16)
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--ordering", choices=["best", "mobility", "learned", "static"], default="best")
+    ap.add_argument("--tt-log2", type=int, default=29,
+                    help="headline solver: shared table of 2^N 16-byte buckets (0 = none; 29 = 8 GiB)")
+    ap.add_argument("--tt-min", type=int, default=8, help="headline solver: probe/store nodes with >= N empties")
+    ap.add_argument("--ppw", type=int, default=16, help="headline solver: boards a wavefront owns at once")
     ap.add_argument("--plain-n", type=int, default=256, help="positions for the plain-kernel leg (0 = skip)")
     ap.add_argument("--plain-limit-s", type=int, default=150, help="time limit of the plain leg")
     ap.add_argument("--cpu-sample", type=int, default=2048,
@@ -85,23 +90,25 @@ def main() -> int:
         else:
             order_name = "mobility"
     ordering = {"static": go2.ORDER_STATIC, "mobility": go2.ORDER_MOBILITY, "learned": go2.ORDER_LEARNED}[order_name]
-    cfg = go2.default_config(ordering=ordering)
+    cfg = go2.default_config(ordering=ordering, tt_log2_buckets=args.tt_log2, tt_min_empties=args.tt_min,
+                             positions_per_wave=args.ppw)
 
     log(f"[bench] generating {n} positions at {E} empties on the GPU")
     boards = go2.generate(args.seed, 4, n, E)
     task = go2.Task(n, E, cfg)
     task.upload(boards)
 
-    # Warmup: the same kernel on a small prefix (code load, clocks); the timed
-    # steps then solve the whole batch.
+    # Warmup: the same task on a small prefix (code load, clocks, the task's
+    # table and stack allocated); the timed steps then solve the whole batch.
+    # Table entries carry a launch generation, so nothing a warmup stored is
+    # visible to a timed launch.
     wn = min(n, 4096)
-    wt = go2.Task(wn, E, cfg)
-    wt.upload(boards[:wn])
+    task.upload(boards[:wn])
     for i in range(args.warmup):
-        wt.launch()
-        wt.wait()
+        task.launch()
+        task.wait()
         log(f"[bench] warmup {i + 1}/{args.warmup} done ({wn} positions)")
-    wt.close()
+    task.upload(boards)
     kernel_ms, nodes = [], []
     t0 = time.perf_counter()
     for i in range(args.steps):
@@ -195,6 +202,8 @@ def main() -> int:
         "config": {
             "workload": f"exact solve of {n} random-playout positions at {E} empties",
             "positions": n, "empties": E, "ordering": order_name,
+            "table_buckets_log2": args.tt_log2, "table_min_empties": args.tt_min,
+            "positions_per_wave": args.ppw,
             "plain_positions": pn, "plain_value": plain_value,
             "headline_value_same_sample": head_small_value,
             "ratio_vs_plain_same_sample": (head_small_value / plain_value) if plain_value else None,
@@ -202,6 +211,7 @@ def main() -> int:
             "ratio_vs_plain_lower_bound": (head_small_value / plain_bound) if plain_bound else None,
             "score_match_plain": plain_match, "score_match_cpu": cpu_match,
             "nodes_per_position": nodes_per_launch / n,
+            "table": {k: st[k] for k in ("tt_probes", "tt_hits", "tt_cuts", "tt_stores")},
             "evaluator": None if learned_meta is None else {
                 k: learned_meta[k] for k in ("positions", "empties", "split", "valid_mse",
                                              "valid_mse_constant_predictor")},
