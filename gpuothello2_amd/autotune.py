@@ -8,6 +8,7 @@ Searched knobs (coordinate descent from the library defaults):
   lds_levels          search-stack levels kept in LDS (the rest in HBM)
   split_min_empties   smallest node helper lanes may split
   tt_log2_buckets     transposition table size (deep batches only)
+  tt_min_empties      smallest node the table serves (deep batches only)
 The result is a plain JSON object of go2_config fields; `apply()` turns it
 into a Config, and the CLI reads the same file through GO2_CONFIG.
 """
@@ -28,7 +29,7 @@ SPACE = {
     "lds_levels": [6, 5, 9],
     "split_min_empties": [8, 7, 10],
 }
-DEEP_SPACE = {"tt_log2_buckets": [0, 22, 26]}
+DEEP_SPACE = {"tt_log2_buckets": [0, 26, 29], "tt_min_empties": [12, 10, 8]}
 
 
 def apply(base: Config | None, params: dict) -> Config:
