@@ -9,6 +9,7 @@
 #include "oracle.h"
 
 #include <limits.h>
+#include <math.h>
 #include <pthread.h>
 #include <stdatomic.h>
 #include <stdlib.h>
@@ -391,6 +392,107 @@ void orc_best_moves_batch(const uint64_t *P, const uint64_t *O, size_t n, int de
   atomic_init(&j.next, 0);
   pthread_t tid[256];
   for (int t = 0; t < threads; ++t) pthread_create(&tid[t], NULL, moves_worker, &j);
+  for (int t = 0; t < threads; ++t) pthread_join(tid[t], NULL);
+}
+
+/* Depth-limited search (the reference's think mode, thinker.cu:120-320):
+ * negamax over `depth` disc placements; a pass does not use up depth
+ * (thinker.cu:137-147,270-276); a game that ends first scores exactly; the
+ * position after the depth-th move is scored by the evaluator, rounded to
+ * whole discs half away from zero and clamped to [-64, 64]
+ * (thinker.cu:305).  Plain recursion without pruning: it defines the value. */
+int orc_think_leaf(const int16_t *w, const int16_t *bias, float scale, uint64_t P, uint64_t O) {
+  float v = roundf((float)orc_eval_int(w, bias, P, O) / scale);
+  if (v > 64.f) v = 64.f;
+  if (v < -64.f) v = -64.f;
+  return (int)v;
+}
+
+static int think_rec(const int16_t *w, const int16_t *bias, float scale, uint64_t P, uint64_t O, int depth) {
+  if (depth == 0) return orc_think_leaf(w, bias, scale, P, O);
+  const uint64_t m = orc_mobility(P, O);
+  if (!m) {
+    if (orc_mobility(O, P)) return -think_rec(w, bias, scale, O, P, depth);
+    return orc_final_score(P, O);
+  }
+  int best = -65;
+  for (uint64_t r = m; r; r &= r - 1) {
+    const int sq = __builtin_ctzll(r);
+    const uint64_t f = orc_flip(P, O, sq);
+    const int v = -think_rec(w, bias, scale, O ^ f, (P ^ f) | (1ULL << sq), depth - 1);
+    if (v > best) best = v;
+  }
+  return best;
+}
+
+int orc_think(const int16_t *w, const int16_t *bias, float scale, uint64_t P, uint64_t O, int depth,
+              uint64_t *optimal, int *kind) {
+  const uint64_t m = orc_mobility(P, O);
+  *optimal = 0;
+  if (!m) {
+    if (orc_mobility(O, P)) {
+      *kind = ORC_MOVES_PASS;
+      return -think_rec(w, bias, scale, O, P, depth);
+    }
+    *kind = ORC_MOVES_GAME_OVER;
+    return orc_final_score(P, O);
+  }
+  *kind = ORC_MOVES_PLAY;
+  int vals[64];
+  int best = -65;
+  for (uint64_t r = m; r; r &= r - 1) {
+    const int sq = __builtin_ctzll(r);
+    const uint64_t f = orc_flip(P, O, sq);
+    vals[sq] = -think_rec(w, bias, scale, O ^ f, (P ^ f) | (1ULL << sq), depth - 1);
+    if (vals[sq] > best) best = vals[sq];
+  }
+  for (uint64_t r = m; r; r &= r - 1)
+    if (vals[__builtin_ctzll(r)] == best) *optimal |= r & (0 - r);
+  return best;
+}
+
+typedef struct {
+  const int16_t *w, *bias;
+  float scale;
+  const uint64_t *P, *O;
+  size_t n;
+  int depth;
+  int8_t *scores;
+  uint64_t *optimal;
+  uint8_t *kinds;
+  atomic_size_t next;
+} think_job;
+
+static void *think_worker(void *arg) {
+  think_job *j = (think_job *)arg;
+  for (;;) {
+    size_t i = atomic_fetch_add(&j->next, 1);
+    if (i >= j->n) break;
+    int kind = 0;
+    j->scores[i] = (int8_t)orc_think(j->w, j->bias, j->scale, j->P[i], j->O[i], j->depth, &j->optimal[i], &kind);
+    j->kinds[i] = (uint8_t)kind;
+  }
+  return NULL;
+}
+
+void orc_think_batch(const int16_t *w, const int16_t *bias, float scale, const uint64_t *P, const uint64_t *O,
+                     size_t n, int depth, int threads, int8_t *scores, uint64_t *optimal, uint8_t *kinds) {
+  if (threads < 1) threads = 1;
+  if (threads > 256) threads = 256;
+  think_job j;
+  j.w = w;
+  j.bias = bias;
+  j.scale = scale;
+  j.P = P;
+  j.O = O;
+  j.n = n;
+  j.depth = depth;
+  j.scores = scores;
+  j.optimal = optimal;
+  j.kinds = kinds;
+  atomic_init(&j.next, 0);
+  pthread_t tid[256];
+  for (int t = 0; t < threads; ++t) pthread_create(&tid[t], NULL, think_worker, &j);
   for (int t = 0; t < threads; ++t) pthread_join(tid[t], NULL);
 }
 

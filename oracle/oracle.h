@@ -61,6 +61,17 @@ int  orc_best_moves(uint64_t P, uint64_t O, int depth, uint64_t *optimal, int *k
 void orc_best_moves_batch(const uint64_t *P, const uint64_t *O, size_t n, int depth, int threads,
                           int8_t *scores, uint64_t *optimal, uint8_t *kinds);
 
+/* Depth-limited search, the reference's think mode (thinker.cu:120-320):
+ * negamax over `depth` (>= 1) disc placements, passes free, exact at game
+ * end, the evaluator (orc_eval_int / scale, rounded half away from zero,
+ * clamped to [-64, 64]) after the depth-th move.  Returns the value; the
+ * optimal squares and kind as orc_best_moves. */
+int  orc_think_leaf(const int16_t *w, const int16_t *bias, float scale, uint64_t P, uint64_t O);
+int  orc_think(const int16_t *w, const int16_t *bias, float scale, uint64_t P, uint64_t O, int depth,
+               uint64_t *optimal, int *kind);
+void orc_think_batch(const int16_t *w, const int16_t *bias, float scale, const uint64_t *P, const uint64_t *O,
+                     size_t n, int depth, int threads, int8_t *scores, uint64_t *optimal, uint8_t *kinds);
+
 /* ---- synthetic positions ---------------------------------------------------- */
 uint64_t orc_splitmix64(uint64_t *state);
 uint64_t orc_position_key(uint64_t seed, uint32_t config, uint64_t index, uint32_t attempt);

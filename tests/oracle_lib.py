@@ -46,6 +46,13 @@ def oracle() -> ctypes.CDLL:
         L.orc_best_moves_batch.argtypes = [ctypes.c_void_p, ctypes.c_void_p, sz, i32, i32, ctypes.c_void_p,
                                            ctypes.c_void_p, ctypes.c_void_p]
         L.orc_best_moves_batch.restype = None
+        L.orc_think_leaf.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float, u64, u64]
+        L.orc_think.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float, u64, u64, i32, P64,
+                                ctypes.POINTER(ctypes.c_int)]
+        L.orc_think_batch.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p,
+                                      ctypes.c_void_p, sz, i32, i32, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p]
+        L.orc_think_batch.restype = None
         L.orc_solve_batch.restype = ctypes.c_double
         L.orc_position_key.argtypes = [u64, u32, u64, u32]
         L.orc_position_key.restype = u64
@@ -153,6 +160,34 @@ def best_moves_batch(boards: np.ndarray, depth: int, threads: int = 8):
     kinds = np.zeros(n, dtype=np.uint8)
     oracle().orc_best_moves_batch(P.ctypes.data, O.ctypes.data, n, depth, threads, scores.ctypes.data,
                                   masks.ctypes.data, kinds.ctypes.data)
+    return scores, masks, kinds
+
+
+def think(w16: np.ndarray, b16: np.ndarray, scale: float, p: int, o: int, depth: int):
+    """(value, mask of optimal squares, kind) of the depth-limited search."""
+    m = u64(0)
+    k = ctypes.c_int(0)
+    v = oracle().orc_think(w16.ctypes.data, b16.ctypes.data, scale, p, o, depth, ctypes.byref(m), ctypes.byref(k))
+    return v, m.value, k.value
+
+
+def think_leaf(w16: np.ndarray, b16: np.ndarray, scale: float, p: int, o: int) -> int:
+    return oracle().orc_think_leaf(w16.ctypes.data, b16.ctypes.data, scale, p, o)
+
+
+def think_batch(w16: np.ndarray, b16: np.ndarray, scale: float, boards: np.ndarray, depth: int, threads: int = 8):
+    """(values, optimal-square masks, kinds) of (n,2) boards."""
+    b = np.ascontiguousarray(boards, dtype=np.uint64).reshape(-1, 2)
+    P = np.ascontiguousarray(b[:, 0])
+    O = np.ascontiguousarray(b[:, 1])
+    n = b.shape[0]
+    scores = np.zeros(n, dtype=np.int8)
+    masks = np.zeros(n, dtype=np.uint64)
+    kinds = np.zeros(n, dtype=np.uint8)
+    w = np.ascontiguousarray(w16, dtype=np.int16)
+    bb = np.ascontiguousarray(b16, dtype=np.int16)
+    oracle().orc_think_batch(w.ctypes.data, bb.ctypes.data, scale, P.ctypes.data, O.ctypes.data, n, depth, threads,
+                             scores.ctypes.data, masks.ctypes.data, kinds.ctypes.data)
     return scores, masks, kinds
 
 
