@@ -41,7 +41,7 @@ ORDER_LEARNED = 2
 EXPORTED_SYMBOLS = (
     "go2_version", "go2_last_error", "go2_set_device", "go2_config_default",
     "go2_board_from_text", "go2_board_to_text", "go2_solve", "go2_solve_device",
-    "go2_solve_moves", "go2_solve_device_moves", "go2_task_download_moves",
+    "go2_solve_moves", "go2_solve_device_moves", "go2_task_download_moves", "go2_think",
     "go2_task_create", "go2_task_upload", "go2_task_launch", "go2_task_query",
     "go2_task_wait", "go2_task_download", "go2_task_destroy", "go2_generate",
     "go2_generate_device", "go2_pattern_info", "go2_pattern_features_device",
@@ -80,7 +80,8 @@ class Config(ctypes.Structure):
         ("graph_launch", ctypes.c_int32),
         ("export_min_empties", ctypes.c_int32),
         ("export_low_empties", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 6),
+        ("think_depth", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 5),
     ]
 
 
@@ -134,6 +135,7 @@ def lib() -> ctypes.CDLL:
     L.go2_solve_moves.argtypes = [vp, sz, i32, vp, vp, ctypes.POINTER(Config), ctypes.POINTER(Stats)]
     L.go2_solve_device_moves.argtypes = [vp, sz, i32, vp, vp, ctypes.POINTER(Config), ctypes.POINTER(Stats), vp]
     L.go2_task_download_moves.argtypes = [vp, vp]
+    L.go2_think.argtypes = [vp, sz, i32, vp, vp, ctypes.POINTER(Config), ctypes.POINTER(Stats)]
     L.go2_task_create.argtypes = [sz, i32, ctypes.POINTER(Config)]
     L.go2_task_create.restype = vp
     L.go2_task_upload.argtypes = [vp, vp, sz]
@@ -216,6 +218,20 @@ def solve_moves(boards: np.ndarray, max_depth: int, config: Config | None = None
     _check(lib().go2_solve_moves(b.ctypes.data if n else None, n, max_depth, scores.ctypes.data,
                                  moves.ctypes.data, ctypes.byref(config) if config is not None else None,
                                  ctypes.byref(st)))
+    return scores[:n], moves[:n], st.as_dict()
+
+
+def think(boards: np.ndarray, depth: int, config: Config | None = None):
+    """The reference's think mode: depth-limited search, the position after
+    `depth` moves scored by the installed evaluator (set_evaluator); boards
+    may have up to 60 empties.  Returns (int8 scores, uint8 moves, stats)."""
+    b = np.ascontiguousarray(boards, dtype=BOARD_DTYPE)
+    n = b.shape[0]
+    scores = np.zeros(max(n, 1), dtype=np.int8)
+    moves = np.zeros(max(n, 1), dtype=np.uint8)
+    st = Stats()
+    _check(lib().go2_think(b.ctypes.data if n else None, n, depth, scores.ctypes.data, moves.ctypes.data,
+                           ctypes.byref(config) if config is not None else None, ctypes.byref(st)))
     return scores[:n], moves[:n], st.as_dict()
 
 

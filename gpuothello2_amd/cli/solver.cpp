@@ -152,15 +152,23 @@ int main(int argc, char** argv) {
     fprintf(stderr, "usage: %s INPUT OUTPUT DEPTH [THINK_DEPTH]\n", argv[0]);
     return 1;
   }
-  if (argc >= 5) die(2, "think mode (THINK_DEPTH) is not provided by this solver");
-
+  // Five arguments select think mode (reference main.cu:175-185, 69-121): a
+  // depth-limited search of THINK_DEPTH moves scored by the evaluator in
+  // GO2_WEIGHTS, each line "<board> <score> <move>".  DEPTH is read but,
+  // as in the reference, not used by think mode.
+  const bool think = argc >= 5;
   char* end = nullptr;
   const long depth = strtol(argv[3], &end, 10);
-  if (!end || *end || depth < 1 || depth > GO2_MAX_EMPTIES)
-    die(2, "DEPTH must be an integer in [1, " + std::to_string(GO2_MAX_EMPTIES) + "]");
+  if (!end || *end || depth < 1 || depth > (think ? 60 : GO2_MAX_EMPTIES))
+    die(2, "DEPTH must be an integer in [1, " + std::to_string(think ? 60 : GO2_MAX_EMPTIES) + "]");
+  long think_depth = 0;
+  if (think) {
+    think_depth = strtol(argv[4], &end, 10);
+    if (!end || *end || think_depth < 1 || think_depth > 30) die(2, "THINK_DEPTH must be an integer in [1, 30]");
+  }
 
   const auto boards = load_boards(argv[1]);
-  for (size_t i = 0; i < boards.size(); ++i) {
+  for (size_t i = 0; i < boards.size() && !think; ++i) {
     const int e = 64 - __builtin_popcountll(boards[i].player | boards[i].opponent);
     if (e > depth) die(2, "a board has " + std::to_string(e) + " empties, more than DEPTH");
   }
@@ -177,12 +185,16 @@ int main(int argc, char** argv) {
     else if (!strcmp(o, "learned")) cfg.ordering = GO2_ORDER_LEARNED;
     else die(2, "GO2_ORDERING must be static, mobility or learned");
   }
+  if (think) {
+    cfg.think_depth = (int)think_depth;
+    cfg.ordering = GO2_ORDER_LEARNED;
+  }
   cfg.block_size = env_int("GO2_BLOCK_SIZE", cfg.block_size);
   cfg.mobility_min_empties = env_int("GO2_MOBILITY_MIN_EMPTIES", cfg.mobility_min_empties);
   cfg.learned_min_empties = env_int("GO2_LEARNED_MIN_EMPTIES", cfg.learned_min_empties);
   if (cfg.ordering == GO2_ORDER_LEARNED) {
     const char* wpath = getenv("GO2_WEIGHTS");
-    if (!wpath) die(2, "GO2_ORDERING=learned needs GO2_WEIGHTS");
+    if (!wpath) die(2, think ? "think mode needs GO2_WEIGHTS (the evaluator)" : "GO2_ORDERING=learned needs GO2_WEIGHTS");
     File wf(fopen(wpath, "rb"));
     if (!wf) die(2, std::string("cannot open GO2_WEIGHTS ") + wpath);
     int ni = 0, nw = 0, nb = 0;
@@ -195,13 +207,13 @@ int main(int argc, char** argv) {
 
   // GO2_BEST_MOVE=1 adds a best move to each output line, written like the
   // reference's think-mode moves (main.cu:31-39,114-117).
-  const bool with_moves = env_int("GO2_BEST_MOVE", 0) != 0;
+  const bool with_moves = think || env_int("GO2_BEST_MOVE", 0) != 0;
   const size_t n = boards.size();
   fprintf(stderr, "n = %zu\n", n);
   std::vector<go2_task*> tasks;
   for (size_t i = 0; i < n; i += kBatch) {
     const size_t cnt = std::min(kBatch, n - i);
-    go2_task* t = go2_task_create(cnt, (int)depth, &cfg);
+    go2_task* t = go2_task_create(cnt, think ? 60 : (int)depth, &cfg);
     if (!t) die(2, go2_last_error());
     if (go2_task_upload(t, boards.data() + i, cnt) != GO2_OK) die(2, go2_last_error());
     tasks.push_back(t);

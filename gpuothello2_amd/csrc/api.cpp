@@ -50,6 +50,7 @@ int resolve_config(const go2_config* cfg, go2_config* out) {
   if (c.time_limit_ms < 0) return fail(GO2_E_ARG, "config.time_limit_ms must be >= 0");
   if (c.export_min_empties < 0) return fail(GO2_E_ARG, "config.export_min_empties must be >= 0");
   if (c.export_low_empties < 0) return fail(GO2_E_ARG, "config.export_low_empties must be >= 0");
+  if (c.think_depth < 0 || c.think_depth > 30) return fail(GO2_E_ARG, "config.think_depth must be in [0, 30]");
   if (c.tt_log2_buckets < 0 || c.tt_log2_buckets > 32)
     return fail(GO2_E_ARG, "config.tt_log2_buckets must be in [0, 32]");
   if (c.split_cut_after < 0 || c.split_cut_after > 3)
@@ -78,9 +79,12 @@ struct IoBufs {
   }
 };
 
-static int check_depth(int max_depth) {
-  if (max_depth < 1 || max_depth > GO2_MAX_EMPTIES)
-    return fail(GO2_E_ARG, "max_depth must be in [1, " + std::to_string(GO2_MAX_EMPTIES) + "]");
+// Largest number of empties a board may have: GO2_MAX_EMPTIES for the exact
+// solve, any position of a game (60) in think mode.
+static int check_depth(int max_depth, const go2_config& c) {
+  const int hi = c.think_depth > 0 ? 60 : GO2_MAX_EMPTIES;
+  if (max_depth < 1 || max_depth > hi)
+    return fail(GO2_E_ARG, "max_depth must be in [1, " + std::to_string(hi) + "]");
   return GO2_OK;
 }
 
@@ -184,10 +188,10 @@ int go2_solve_device(const go2_board* d_boards, size_t n, int max_depth, int8_t*
 int go2_solve_device_moves(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_scores,
                            uint8_t* d_moves, const go2_config* cfg, go2_stats* stats, void* hip_stream) {
   int rc;
-  if ((rc = check_depth(max_depth)) != GO2_OK) return rc;
   if (n && (!d_boards || !d_scores)) return fail(GO2_E_ARG, "null device array");
   go2_config c;
   if ((rc = resolve_config(cfg, &c)) != GO2_OK) return rc;
+  if ((rc = check_depth(max_depth, c)) != GO2_OK) return rc;
   hipStream_t s = (hipStream_t)hip_stream;
   if (n == 0) {
     if (stats) *stats = go2_stats{};
@@ -219,11 +223,10 @@ int go2_solve_moves(const go2_board* boards, size_t n, int max_depth, int8_t* sc
 }
 
 go2_task* go2_task_create(size_t n, int max_depth, const go2_config* cfg) {
-  if (check_depth(max_depth) != GO2_OK) return nullptr;
   go2_task* t = new go2_task;
   t->n = n;
   t->max_depth = max_depth;
-  if (resolve_config(cfg, &t->cfg) != GO2_OK) {
+  if (resolve_config(cfg, &t->cfg) != GO2_OK || check_depth(max_depth, t->cfg) != GO2_OK) {
     delete t;
     return nullptr;
   }
@@ -297,6 +300,17 @@ int go2_task_download(go2_task* t, int8_t* scores, go2_stats* stats) {
   if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(scores)");
   if ((e = hipStreamSynchronize(t->stream)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
   return GO2_OK;
+}
+
+int go2_think(const go2_board* boards, size_t n, int depth, int8_t* scores, uint8_t* moves,
+              const go2_config* cfg, go2_stats* stats) {
+  if (depth < 1 || depth > 30) return fail(GO2_E_ARG, "think depth must be in [1, 30]");
+  go2_config c;
+  int rc;
+  if ((rc = resolve_config(cfg, &c)) != GO2_OK) return rc;
+  c.think_depth = depth;
+  c.ordering = GO2_ORDER_LEARNED;
+  return go2_solve_moves(boards, n, 60, scores, moves, &c, stats);
 }
 
 int go2_task_download_moves(go2_task* t, uint8_t* moves) {

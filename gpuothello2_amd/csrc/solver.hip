@@ -113,6 +113,37 @@ __device__ __forceinline__ int frame_val(uint32_t w, int shift) {
   return ((int)(w << (25 - shift)) >> 25) * 2;
 }
 
+// Think mode's values need not be even (rounded evaluations) and it has no
+// helpers, so its frame word keeps alpha + 64 (bits 0-7), beta + 64 (8-15)
+// and best + 64 (low five bits in 16-20, high three in 28-30) instead of the
+// split bookkeeping; the move square and pass flag stay at 21-27.
+template <bool THINK>
+__device__ __forceinline__ uint32_t frame_word(int alpha, int beta, int best, int sq, bool passed, int done,
+                                               int type) {
+  if constexpr (THINK) {
+    const uint32_t b = (uint32_t)(best + 64);
+    return (uint32_t)(alpha + 64) | ((uint32_t)(beta + 64) << 8) | ((b & 31u) << 16) | ((uint32_t)sq << 21) |
+           ((uint32_t)passed << 27) | ((b >> 5) << 28);
+  } else {
+    return pack_frame(alpha, beta, best, sq, passed, done, type);
+  }
+}
+template <bool THINK>
+__device__ __forceinline__ int fw_alpha(uint32_t w) {
+  if constexpr (THINK) return (int)(w & 255u) - 64;
+  else return frame_val(w, 0);
+}
+template <bool THINK>
+__device__ __forceinline__ int fw_beta(uint32_t w) {
+  if constexpr (THINK) return (int)((w >> 8) & 255u) - 64;
+  else return frame_val(w, 7);
+}
+template <bool THINK>
+__device__ __forceinline__ int fw_best(uint32_t w) {
+  if constexpr (THINK) return (int)(((w >> 16) & 31u) | (((w >> 28) & 7u) << 5)) - 64;
+  else return frame_val(w, 14);
+}
+
 // Expected type of a child: the first child of a PV node is PV, its other
 // children are CUT nodes; children of CUT nodes are ALL nodes and children
 // of ALL nodes are CUT nodes.
@@ -126,14 +157,14 @@ constexpr int kNoValue = -127;  // "no helper reported yet"
 constexpr int kFrameBase = 4;
 
 // A child value on its way into a node, with the move that produced it:
-// ((value + 64) | exact) << 6 | square.  Values are even, so bit 0 of the
-// shifted field says whether the value is exact (it beat the alpha the child
-// was searched with); the largest key over a node's children names a move
-// that reaches the node's final value.  Negative = none.
+// ((value + 64) << 1 | exact) << 6 | square, where exact says the value beat
+// the alpha the child was searched with; the largest key over a node's
+// children names a move that reaches the node's final value.  Negative =
+// none.
 __device__ __forceinline__ int move_key(int v, bool exact, int sq) {
-  return (((v + 64) | (int)exact) << 6) | sq;
+  return ((((v + 64) << 1) | (int)exact) << 6) | sq;
 }
-__device__ __forceinline__ int key_value(int k) { return ((k >> 6) & ~1) - 64; }
+__device__ __forceinline__ int key_value(int k) { return (k >> 7) - 64; }
 constexpr int kMovePass = 64, kMoveNone = 65;
 
 struct SolveArgs {
@@ -157,6 +188,8 @@ struct SolveArgs {
   int key_eval_mul;
   int key_mob_mul;
   int max_pops;       // finished nodes a lane returns from per loop trip
+  int think_depth;    // think mode: moves before the evaluator scores a position
+  float ev_scale;     // evaluator units per disc
   // transposition table (null = none): probed when pushing children with
   // >= tt_min empties, written when such nodes finish
   ulonglong2* tt;
@@ -289,8 +322,14 @@ __device__ __forceinline__ void undo_move(uint64_t& P, uint64_t& O, uint64_t fb,
   }
 }
 
-template <int MAXE, int ORDER>
+// THINK: the reference's think mode (thinker.cu:120-320) instead of the exact
+// solve: the position after the think_depth-th move from a board is scored
+// by the evaluator (rounded to whole discs); passes cost no depth.  One lane
+// per board (no helpers, no table: stored values would depend on each
+// board's horizon), learned ordering.
+template <int MAXE, int ORDER, bool THINK = false>
 __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
+  static_assert(!THINK || ORDER == 2, "think mode needs the evaluator's tables");
   extern __shared__ uint64_t lds[];
   const int bs = blockDim.x, tid = threadIdx.x;
   const int lane = __lane_id();
@@ -320,9 +359,13 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
   // Every frame slot is read and written only by the lane that owns it.
   // LDS levels use ds_* accesses and HBM levels global ones (no flat
   // addressing on the hot path).
-  auto lds_k = [&](int E) { return (E - kFrameBase) * bs + tid; };
-  auto hbm_k = [&](int E) { return (size_t)(E - kFrameBase - LL) * a.total_lanes + gtid; };
-  auto in_lds = [&](int E) { return E - kFrameBase < LL; };
+  // Frames are indexed by E - ebase: ebase is 0 for the exact solve; in
+  // think mode it is (horizon empties - 2) of the lane's board, so the
+  // frames a search can push are levels 4..think_depth+2.
+  int ebase = 0;
+  auto lds_k = [&](int E) { return (E - ebase - kFrameBase) * bs + tid; };
+  auto hbm_k = [&](int E) { return (size_t)(E - ebase - kFrameBase - LL) * a.total_lanes + gtid; };
+  auto in_lds = [&](int E) { return E - ebase - kFrameBase < LL; };
   auto get_sx = [&](int E) -> uint32_t { return in_lds(E) ? l_sx[lds_k(E)] : hbm_ld(&a.g_sx[hbm_k(E)]); };
   auto set_sx = [&](int E, uint32_t v) {
     if (in_lds(E)) {
@@ -518,13 +561,13 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
               atomicCAS(&a.ctr->error, 0u, i + 1);
               a.scores[i] = 0;
               if (a.moves) a.moves[i] = (uint8_t)kMoveNone;
-            } else if (E <= 1) {
+            } else if (E <= 1 && (!THINK || a.think_depth > E)) {
               const int sq = ctz64(~(P | O));
               a.scores[i] = (int8_t)(E == 0 ? final_score(P, O) : last_score(P, O, sq));
               if (a.moves)
                 a.moves[i] = (uint8_t)(E == 0 ? kMoveNone
                                               : flip(P, O, sq) ? sq : flip(O, P, sq) ? kMovePass : kMoveNone);
-            } else if (E == 2) {
+            } else if (E == 2 && (!THINK || a.think_depth > 2)) {
               int nl = 0, mv = kMoveNone;
               a.scores[i] = (int8_t)solve2(P, O, lastc, nl, &mv);
               c_nodes += nl;
@@ -553,6 +596,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
                 beta = 64;
                 owner = -1;
                 mkey = -1;
+                if constexpr (THINK) ebase = E - a.think_depth - 2;
                 enter_node(m, kPV);
                 mode = SEARCH;
               }
@@ -746,7 +790,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
         uint64_t mv = get_mv(e), od = get_od(e);
         const uint32_t w = get_wd(e);
         const uint32_t sx = get_sx(e);
-        const int fa = frame_val(w, 0), fbeta = frame_val(w, 7);
+        const int fa = fw_alpha<THINK>(w), fbeta = fw_beta<THINK>(w);
         const int am = max(fa, sx_best(sx));
         if (mv == 0 || am >= fbeta) {
           split_mask &= ~(1u << (e - 3));
@@ -803,7 +847,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
         uint64_t mv = get_mv(e), od = get_od(e);
         const uint32_t w = get_wd(e);
         const uint32_t sx = get_sx(e);
-        const int fa = frame_val(w, 0), fbeta = frame_val(w, 7);
+        const int fa = fw_alpha<THINK>(w), fbeta = fw_beta<THINK>(w);
         const int am = max(fa, sx_best(sx));
         // a free record for taking the sibling back (records of claimed
         // siblings are dropped when their reports arrive)
@@ -878,7 +922,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
       ++c_iters;
       int sq;
       uint64_t f, mx, my;
-      bool leaf = false;
+      bool leaf = false, tleaf = false;
       if (pchk) {
         sq = pchk_sq;
         f = pchk_f;
@@ -892,14 +936,29 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
           sq = pick_static<ORDER>(moves, ~(P | O));
         }
         f = flip(P, O, sq);
-        leaf = !scoring && E == 3;
+        if constexpr (THINK) {
+          tleaf = !scoring && E - 1 == ebase + 2;               // the child is at the horizon
+          // exact below only when the horizon lies beyond the end: a
+          // position reached by the depth-th move is scored by the evaluator
+          // even when the board is full
+          leaf = !scoring && E == 3 && ebase + 2 < 0 && !tleaf;
+        } else {
+          leaf = !scoring && E == 3;
+        }
       }
       const uint64_t bit = 1ULL << sq;
       const uint64_t cP = O ^ f, cO = P ^ f ^ bit;
       mx = pchk ? cO : cP;
       my = pchk ? cP : cO;
       uint64_t cm = 0;
-      if (leaf) {
+      if (THINK && tleaf) {
+        if constexpr (ORDER == 2) {
+          int ev = a.ev_bias[E - 1];
+          for_each_feature(cP, cO, t3, [&](int, uint32_t fi) { ev += a.ev_w[fi]; });
+          const float r = roundf((float)ev / a.ev_scale);
+          v = -(int)fminf(fmaxf(r, -64.f), 64.f);
+        }
+      } else if (leaf) {
         // the child has two empty squares: its exact value in this trip
         int nl = 0;
         v = -solve2(cP, cO, lastc, nl);
@@ -942,7 +1001,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
       } else {
         moves ^= bit;
         ++c_nodes;
-        if (leaf) {
+        if (leaf || tleaf) {
           commit = true;
         } else if (!cm) {
           pchk = true;
@@ -983,7 +1042,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
         }
       }
       if (push) {
-        push_frame(E, f | bit, moves, order, pack_frame(alpha, beta, best, sq, passed, ndone, ntype));
+        push_frame(E, f | bit, moves, order, frame_word<THINK>(alpha, beta, best, sq, passed, ndone, ntype));
         // Young-brothers-wait: siblings are offered after the first child
         // (after cut_after children at expected CUT nodes).  When nearly
         // every wavefront is idle, speculation costs little and the serial
@@ -1099,10 +1158,10 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
           order = hbm_ld(&a.g_od[k]);
           w = hbm_ld(&a.g_wd[k]);
         }
-        split_mask &= ~(1u << (E - 3));
-        const int palpha = frame_val(w, 0);
-        const int pbeta = frame_val(w, 7);
-        const int pbest = frame_val(w, 14);
+        if constexpr (!THINK) split_mask &= ~(1u << (E - 3));
+        const int palpha = fw_alpha<THINK>(w);
+        const int pbeta = fw_beta<THINK>(w);
+        const int pbest = fw_best<THINK>(w);
         undo_move(P, O, fb, w);
         const int vv = ((w >> 27) & 1) ? best : -best;
         if (E == Eroot && owner < 0) mkey = max(mkey, move_key(vv, vv > palpha, (int)((w >> 21) & 63)));
@@ -1113,8 +1172,13 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
           const uint32_t psx = get_sx(E);
           a0mv = (((sx_alpha0(psx) >> 1) & 127) << 6) | (vv > pbest ? (int)((w >> 21) & 63) : (int)((psx >> 23) & 63));
         }
-        ndone = min((int)((w >> 28) & 3) + 1, 3);
-        ntype = (int)(w >> 30);
+        if constexpr (THINK) {
+          ndone = 1;
+          ntype = kAll;
+        } else {
+          ndone = min((int)((w >> 28) & 3) + 1, 3);
+          ntype = (int)(w >> 30);
+        }
         scoring = false;
       }
     }
@@ -1298,15 +1362,20 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
                  hipEvent_t ev1, GraphCache* graph) {
   if (n > 0xFFFFFF00ULL) return fail(GO2_E_ARG, "too many boards in one solve (max 2^32-256)");
   KernelFn fn = nullptr;
-  const int maxe = stack_levels_for(max_depth, &fn, cfg.ordering);
+  const bool think = cfg.think_depth > 0;
+  // Think mode pushes frames for levels 4..think_depth+2 (relative to each
+  // board's horizon); the exact solve for levels 4..maxe.
+  const int maxe = think ? cfg.think_depth + 2 : stack_levels_for(max_depth, &fn, cfg.ordering);
+  if (think) fn = solve_kernel<32, 2, true>;
   const int bs = cfg.block_size;
-  const int levels = maxe + 1 - kFrameBase;  // frame levels 4..maxe
+  const int levels = max(1, maxe + 1 - kFrameBase);
   // Keep the stack in LDS up to 9 empties: three 256-lane blocks per CU (the
   // kernel's register budget allows three wavefronts per SIMD) with the
   // last-square and base-3 tables; nodes nearer the root use HBM frames.
   int lds_levels = cfg.lds_levels > 0 ? cfg.lds_levels : 6;
   if (lds_levels > levels) lds_levels = levels;
-  const size_t lds = (size_t)lds_levels * bs * 32 + 2048 + (cfg.ordering == GO2_ORDER_LEARNED ? kBase3Size * 2 : 0);
+  const size_t lds = (size_t)lds_levels * bs * 32 + 2048 +
+                     (cfg.ordering == GO2_ORDER_LEARNED || think ? kBase3Size * 2 : 0);
   hipError_t e;
   if (lds > 65536) {
     e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -1328,12 +1397,13 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   int split_min = cfg.split_min_empties > 0 ? cfg.split_min_empties : 8;
   if (split_min < 5) split_min = 5;
   if (split_min > maxe + 1) split_min = maxe + 1;  // no split levels at all
+  if (think) split_min = 64;                       // one lane per board
   const int ext_min = cfg.export_min_empties > 0 ? max(cfg.export_min_empties, split_min) : max(12, split_min);
   const int ext_lo = min(ext_min, max(split_min, cfg.export_low_empties > 0 ? cfg.export_low_empties : 9));
   // Without cross-wavefront help, never launch more wavefronts than boards;
   // with it, the whole resident grid runs and idle wavefronts take siblings.
   // Slot tags hold global lane ids below 2^19.
-  const bool help = cfg.export_min_empties <= 60 && ext_lo <= maxe &&
+  const bool help = !think && cfg.export_min_empties <= 60 && ext_lo <= maxe &&
                     (size_t)cus * per_cu * bs < (size_t(1) << 19);
   const size_t max_grid = (n + waves_per_block - 1) / waves_per_block;
   if (!help && grid > max_grid) grid = max_grid;
@@ -1356,7 +1426,8 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   args.scores = d_scores;
   args.moves = d_moves;
   args.n = (unsigned int)n;
-  args.depth = max_depth;
+  args.depth = think ? 60 : max_depth;
+  args.think_depth = cfg.think_depth;
   args.kff = cfg.mobility_min_empties;
   args.split_min = split_min;
   args.cut_after = cfg.split_cut_after > 0 ? cfg.split_cut_after : 3;
@@ -1387,7 +1458,7 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   } else {
     args.ext_min = args.ext_lo = 3;
   }
-  if (cfg.tt_log2_buckets > 0) {
+  if (cfg.tt_log2_buckets > 0 && !think) {
     if ((rc = ws.ensure_tt(cfg.tt_log2_buckets, stream)) != GO2_OK) return rc;
     if ((rc = zobrist_table(dev, &args.zob)) != GO2_OK) return rc;
     args.tt = reinterpret_cast<ulonglong2*>(ws.tt);
@@ -1395,9 +1466,12 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
     args.tt_gen = &ws.ctr->tt_gen;
     args.tt_min = max(max(cfg.tt_min_empties > 0 ? cfg.tt_min_empties : 12, split_min), 4);
   }
-  if (cfg.ordering == GO2_ORDER_LEARNED) {
+  if (cfg.ordering == GO2_ORDER_LEARNED || think) {
     const EvalTables& t = g_eval[dev];
-    if (!t.w) return fail(GO2_E_STATE, "GO2_ORDER_LEARNED needs go2_set_evaluator() on this device");
+    if (!t.w)
+      return fail(GO2_E_STATE, think ? "think mode needs go2_set_evaluator() on this device"
+                                     : "GO2_ORDER_LEARNED needs go2_set_evaluator() on this device");
+    args.ev_scale = t.scale;
     args.ev_w = t.w;
     args.ev_bias = t.bias;
     args.learned_min = cfg.learned_min_empties;

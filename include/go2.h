@@ -84,7 +84,10 @@ typedef struct go2_config {
   int32_t export_low_empties;   /* the export level falls linearly from export_min_empties to this
                                    one as the idle fraction of wavefronts approaches one; 0 = auto
                                    (9), never below split_min_empties */
-  int32_t reserved[6];
+  int32_t think_depth;          /* > 0: the reference's think mode instead of the exact solve: the
+                                   position after this many moves is scored by the installed
+                                   evaluator (go2_think) */
+  int32_t reserved[5];
 } go2_config;
 
 /* Counters of one solve.  Replaces the reference's `total` node counter
@@ -145,6 +148,18 @@ int go2_solve_moves(const go2_board *boards, size_t n, int max_depth, int8_t *sc
 int go2_solve_device_moves(const go2_board *d_boards, size_t n, int max_depth, int8_t *d_scores,
                            uint8_t *d_moves, const go2_config *cfg, go2_stats *stats,
                            void *hip_stream);
+
+/* Think mode (replaces BatchedThinkTask, reference thinker.cu:351-389 and
+ * main.cu:69-121): depth-limited negamax over `depth` (1..30) moves from each
+ * board (passes cost no depth; a game that ends first scores exactly), the
+ * position after the depth-th move scored by the evaluator installed with
+ * go2_set_evaluator (rounded to whole discs, clamped to [-64, 64]).  Boards
+ * may have up to 60 empties.  scores[i] and moves[i] (may be NULL) as
+ * go2_solve_moves.  The reference's own weights (value/value52) are not
+ * shipped with it, so its think-mode numbers cannot be reproduced; the
+ * search rule is the same. */
+int go2_think(const go2_board *boards, size_t n, int depth, int8_t *scores, uint8_t *moves,
+              const go2_config *cfg, go2_stats *stats);
 
 /* ---- batched task (replaces BatchedTask, reference solver.cuh:8-25,
  *      solver.cu:313-348) ------------------------------------------------------------ */
