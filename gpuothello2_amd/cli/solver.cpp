@@ -15,8 +15,11 @@
 // Undefined behaviour of the reference becomes a defined error (exit code 2):
 // unreadable files, a non-numeric or out-of-range DEPTH, a board with more
 // empties than DEPTH, malformed characters, overlapping discs, device errors.
-// Think mode (five arguments, main.cu:180-183) is not part of this solver and
-// is rejected with exit code 2.
+// Five arguments select think mode (main.cu:180-183): a depth-limited search
+// of THINK_DEPTH moves scored by the evaluator in GO2_WEIGHTS, each output
+// line "<base81> <score> <move>" (main.cu:114-117).
+// Board files are read, sorted and written by the multi-threaded routines
+// of board_io.hpp, with the reference's reading rules.
 //
 // Search knobs come from the environment: GO2_CONFIG=file.json (integer
 // go2_config fields, as the autotuner writes them), then GO2_ORDERING=
@@ -25,6 +28,8 @@
 // GO2_ORDERING=learned needs GO2_WEIGHTS=<raw file: 167265 int16 weights,
 // 64 int16 biases, little endian> (scale 64 per disc).
 #include <go2.h>
+
+#include "board_io.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -48,40 +53,7 @@ using File = std::unique_ptr<FILE, FileCloser>;
 
 constexpr size_t kBatch = 2000000;
 
-[[noreturn]] void die(int code, const std::string& msg) {
-  fprintf(stderr, "error: %s\n", msg.c_str());
-  exit(code);
-}
-
-bool board_less(const go2_board& a, const go2_board& b) {
-  return a.player != b.player ? a.player < b.player : a.opponent < b.opponent;
-}
-bool board_eq(const go2_board& a, const go2_board& b) {
-  return a.player == b.player && a.opponent == b.opponent;
-}
-
-std::vector<go2_board> load_boards(const char* path) {
-  File in(fopen(path, "r"));
-  if (!in) die(2, std::string("cannot open input ") + path);
-  long long n = -1;
-  if (fscanf(in.get(), "%lld\n", &n) != 1 || n < 0) die(2, "input does not start with a board count");
-  std::vector<go2_board> v((size_t)n);
-  for (long long i = 0; i < n; ++i) {
-    char buf[33] = {0};
-    if (!fgets(buf, 32, in.get())) {
-      fprintf(stderr, "Failed to load boards\n");
-      exit(EXIT_FAILURE);
-    }
-    if (strnlen(buf, 32) < 16) die(2, "line " + std::to_string(i + 2) + " is shorter than 16 characters");
-    if (go2_board_from_text(buf, &v[(size_t)i]) != GO2_OK)
-      die(2, "line " + std::to_string(i + 2) + ": " + go2_last_error());
-    if (v[(size_t)i].player & v[(size_t)i].opponent)
-      die(2, "line " + std::to_string(i + 2) + ": a square holds both colours");
-  }
-  std::sort(v.begin(), v.end(), board_less);
-  v.erase(std::unique(v.begin(), v.end(), board_eq), v.end());
-  return v;
-}
+using go2cli::die;
 
 // Integer fields of a flat JSON object {"name": value, ...}.
 void apply_json_config(const char* path, go2_config* c) {
@@ -167,7 +139,7 @@ int main(int argc, char** argv) {
     if (!end || *end || think_depth < 1 || think_depth > 30) die(2, "THINK_DEPTH must be an integer in [1, 30]");
   }
 
-  const auto boards = load_boards(argv[1]);
+  const auto boards = go2cli::load_boards(argv[1]);
   for (size_t i = 0; i < boards.size() && !think; ++i) {
     const int e = 64 - __builtin_popcountll(boards[i].player | boards[i].opponent);
     if (e > depth) die(2, "a board has " + std::to_string(e) + " empties, more than DEPTH");
@@ -248,14 +220,7 @@ int main(int argc, char** argv) {
   fprintf(stderr,
           "hipSuccess (no error), elapsed: %ldms, table update count: %d, table hit: %d, table find: %d\n",
           ms, 0, 0, 0);
-  char buf[17];
-  for (size_t j = 0; j < n; ++j) {
-    go2_board_to_text(&boards[j], buf);
-    if (with_moves)
-      fprintf(out.get(), "%s %d %s\n", buf, (int)scores[j], move_text(moves[j]).c_str());
-    else
-      fprintf(out.get(), "%s %d\n", buf, (int)scores[j]);
-  }
+  go2cli::write_results(out.get(), boards, scores.data(), with_moves ? moves.data() : nullptr, move_text);
   fprintf(stderr, "total nodes: %" PRIu64 "\n", nodes);
   return 0;
 }
