@@ -32,13 +32,25 @@
 //     by the rare nodes near the root).  A frame holds the flip set of the
 //     move taken (the parent board is rebuilt from the child by XORs), the
 //     untried moves, the packed move order and alpha/beta/best/move/flags.
-//     Nodes at split levels also keep their board, a pending-helper count
-//     and the best value helpers reported, in HBM.
+//     Frames at split levels add a split word: helpers still out, the best
+//     value they reported, the node's entry alpha and best move (for the
+//     table) and the siblings out with other wavefronts.
 //   * Every loop iteration does the same heavy work on every searching
 //     lane: one flip and one mobility (or, at nodes with three empties, the
 //     exact value of the two-empties child in straight-line code, so nodes
-//     with two empties never become frames).  Move ordering work is spread over
-//     iterations ("scoring" steps: one child per iteration).
+//     with two empties never become frames).  Move ordering work is spread
+//     over iterations ("scoring" steps: one child per iteration).
+//   * Across wavefronts: while some wavefront is idle, busy ones export
+//     siblings of high nodes to help slots (more, and lower, as the idle
+//     fraction grows); idle wavefronts claim them and report through
+//     per-frame mailboxes; owners take back unclaimed siblings.  Wavefronts
+//     count themselves active when they start and leave once none is.
+//   * Optional lock-free transposition table (Zobrist keys, XOR-checked
+//     16-byte buckets, launch generations), probed when children are pushed.
+//   * Best moves: values reaching a board's root carry the move and an
+//     "exact" flag; the largest such key names an optimal move.
+//   * THINK instantiation: the reference's think mode, a depth-limited
+//     search scored by the pattern evaluator at its horizon.
 //   * Exact fail-soft negamax, full window (-64, 64) at the root; passes
 //     are folded into the child frame (board swapped, window kept, value not
 //     negated), the recurrence of solver.cu:65-73,162-165,214-221,246-256.
