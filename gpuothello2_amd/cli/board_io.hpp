@@ -102,7 +102,13 @@ inline std::vector<go2_board> load_boards(const char* path) {
   if (num.empty() || !end || *end || n < 0) die(2, "input does not start with a board count");
   while (pos < len && isspace((unsigned char)s[pos])) ++pos;  // the "\n" of "%lld\n"
 
-  // Line starts, the way fgets(buf, 32) cuts the stream.
+  // Line starts, the way fgets(buf, 32) cuts the stream.  Every line holds
+  // at least one byte, so a count beyond the file's size fails here, before
+  // anything of the count's size is allocated.
+  if ((unsigned long long)n > len - pos) {
+    fprintf(stderr, "Failed to load boards\n");
+    exit(EXIT_FAILURE);
+  }
   std::vector<size_t> start((size_t)n), size((size_t)n);
   for (long long i = 0; i < n; ++i) {
     if (pos >= len) {

@@ -70,6 +70,8 @@ def test_errors(tmp_path):
     assert r.returncode == 2 and "count" in r.stderr
     r, out = run(tmp_path, "0\n")
     assert r.returncode == 0 and out == ""
+    r, _ = run(tmp_path, "1000000000000000\n" + e + "\n")  # a count far beyond the file
+    assert r.returncode == 1 and "Failed to load boards" in r.stderr
 
 
 def test_two_million_lines(tmp_path):
