@@ -121,7 +121,7 @@ struct go2_task {
 
 extern "C" {
 
-int go2_version(void) { return 1; }
+int go2_version(void) { return 2; }
 
 const char* go2_last_error(void) { return g_last_error.c_str(); }
 

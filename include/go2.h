@@ -109,6 +109,8 @@ typedef struct go2_stats {
 } go2_stats;
 
 /* ---- library ---------------------------------------------------------------- */
+/* 2: best moves (go2_solve_moves, go2_solve_device_moves,
+ *    go2_task_download_moves), think mode (go2_think, config.think_depth). */
 int go2_version(void);
 const char *go2_last_error(void);
 /* Select the HIP device for the calling thread's subsequent calls. */
