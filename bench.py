@@ -59,6 +59,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=2048,
                     help="positions for the CPU baseline leg (about 15 s on 16 threads at 16 empties)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
+    ap.add_argument("--cpu-single-sample", type=int, default=128,
+                    help="positions for the one-thread CPU run (0 = skip)")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES device-to-device (HBM counter calibration for rocprofv3)")
@@ -175,7 +177,20 @@ def main() -> int:
         b2 = np.stack([boards["player"][:k], boards["opponent"][:k]], 1)
         cs, cnodes, secs = ol.solve_batch(b2, E, threads=threads)
         cpu_match = bool(np.array_equal(cs, scores[:k]))
+        k1 = min(args.cpu_single_sample, k)
+        single = None
+        if k1 > 0:
+            c1, _, secs1 = ol.solve_batch(b2[:k1], E, threads=1)
+            cpu_match = cpu_match and bool(np.array_equal(c1, scores[:k1]))
+            single = {"value": k1 / secs1, "positions": k1, "seconds": secs1}
+        model = None
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), None)
+        except OSError:
+            pass
         cpu = {"value": k / secs, "unit": "positions/s", "cores": threads, "kind": "port",
+               "cpu_model": model, "host_cores": os.cpu_count(), "single_thread": single,
                "sample": f"first {k} of the same {E}-empty positions, reference method "
                          f"(fastest-first in the top DEPTH-9 plies, corner/LSB order below, no table), "
                          f"{threads} threads, {secs:.1f} s"}
