@@ -56,6 +56,7 @@
 //     negated), the recurrence of solver.cu:65-73,162-165,214-221,246-256.
 //   * A wall-clock deadline (s_memrealtime) bounds every launch.
 #include <cstring>
+#include <mutex>
 
 #include "bitboard.hpp"
 #include "internal.hpp"
@@ -1241,6 +1242,8 @@ struct EvalTables {
   float scale = 0.f;
 };
 static EvalTables g_eval[64];
+// Guards the per-device evaluator tables and Zobrist keys while they are set up.
+static std::mutex g_zob_mutex;
 
 }  // namespace go2
 
@@ -1251,6 +1254,7 @@ extern "C" int go2_set_evaluator(const int16_t* w, const int16_t* bias, float sc
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
   if (dev < 0 || dev >= 64) return fail(GO2_E_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_zob_mutex);
   EvalTables& t = g_eval[dev];
   if (!t.w && (e = hipMalloc(&t.w, kPatWeights * sizeof(int16_t))) != hipSuccess)
     return hip_fail(e, "hipMalloc(evaluator)");
@@ -1315,7 +1319,11 @@ int Workspace::ensure_tt(int log2_buckets, hipStream_t stream) {
 // Zobrist keys per device: 16 byte positions x 256 byte values, SplitMix64.
 static uint64_t* g_zob[64];
 
+// The keys of one device, built once; the pointer is published only after
+// the keys are on the device, under a lock (tasks may launch from several
+// host threads).
 static int zobrist_table(int dev, const uint64_t** out) {
+  std::lock_guard<std::mutex> lock(g_zob_mutex);
   if (!g_zob[dev]) {
     uint64_t h[16 * 256];
     uint64_t st = 0x5EED0F2A11CE5EEDULL;
@@ -1327,9 +1335,13 @@ static int zobrist_table(int dev, const uint64_t** out) {
       h[i] = z ^ (z >> 31);
     }
     hipError_t e;
-    if ((e = hipMalloc(&g_zob[dev], sizeof(h))) != hipSuccess) return hip_fail(e, "hipMalloc(zobrist)");
-    if ((e = hipMemcpy(g_zob[dev], h, sizeof(h), hipMemcpyHostToDevice)) != hipSuccess)
+    uint64_t* d = nullptr;
+    if ((e = hipMalloc(&d, sizeof(h))) != hipSuccess) return hip_fail(e, "hipMalloc(zobrist)");
+    if ((e = hipMemcpy(d, h, sizeof(h), hipMemcpyHostToDevice)) != hipSuccess) {
+      (void)hipFree(d);
       return hip_fail(e, "hipMemcpy(zobrist)");
+    }
+    g_zob[dev] = d;
   }
   *out = g_zob[dev];
   return GO2_OK;

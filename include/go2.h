@@ -65,7 +65,8 @@ typedef struct go2_config {
                                    Lanes without a board of their own help the others. */
   int32_t split_min_empties;    /* idle lanes take sibling subtrees of nodes with >= this many
                                    empties (young-brothers-wait); 0 = auto, >60 = never */
-  int32_t lds_levels;           /* stack levels (3, 4, ...) kept in LDS, the rest in HBM; 0 = auto */
+  int32_t lds_levels;           /* stack levels (from 4 empties up) kept in LDS, the rest in HBM;
+                                   0 = auto (6) */
   int32_t time_limit_ms;        /* abandon a launch after this long (GO2_E_TIMEOUT); 0 = no limit */
   int32_t split_cut_after;      /* at nodes expected to cut, helpers wait for this many children
                                    (1..3) instead of one; 0 = auto */
@@ -233,8 +234,10 @@ int go2_sgd_apply_device(float *d_w, float *d_bias, float *d_grad, uint32_t *d_c
 int go2_quantize_weights_device(const float *d_w, const float *d_bias, float scale, int16_t *d_w16,
                                 int16_t *d_bias16, void *hip_stream);
 /* Install the int16 evaluator (host arrays of n_weights and n_bias entries;
- * `scale` = units per disc) used by GO2_ORDER_LEARNED solves on the current
- * device.  Without one, GO2_ORDER_LEARNED solves fail with GO2_E_STATE. */
+ * `scale` = units per disc) used by GO2_ORDER_LEARNED solves and think mode
+ * on the current device.  Without one, those fail with GO2_E_STATE.  The
+ * tables are replaced in place: call it before launching such solves, not
+ * while any of them runs on the device. */
 int go2_set_evaluator(const int16_t *w, const int16_t *bias, float scale);
 
 #ifdef __cplusplus
