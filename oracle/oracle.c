@@ -425,6 +425,33 @@ static int think_rec(const int16_t *w, const int16_t *bias, float scale, uint64_
   return best;
 }
 
+/* The same value by fail-soft alpha-beta (window (-65, 65) at the root),
+ * for depths the plain recursion cannot reach; equal to think_rec by the
+ * alpha-beta theorem and checked against it by the tests. */
+static int think_ab(const int16_t *w, const int16_t *bias, float scale, uint64_t P, uint64_t O, int depth,
+                    int alpha, int beta) {
+  if (depth == 0) return orc_think_leaf(w, bias, scale, P, O);
+  const uint64_t m = orc_mobility(P, O);
+  if (!m) {
+    if (orc_mobility(O, P)) return -think_ab(w, bias, scale, O, P, depth, -beta, -alpha);
+    return orc_final_score(P, O);
+  }
+  int best = -65;
+  for (uint64_t r = m; r; r &= r - 1) {
+    const int sq = __builtin_ctzll(r);
+    const uint64_t f = orc_flip(P, O, sq);
+    const int v = -think_ab(w, bias, scale, O ^ f, (P ^ f) | (1ULL << sq), depth - 1, -beta,
+                            -(alpha > best ? alpha : best));
+    if (v > best) best = v;
+    if (best >= beta) break;
+  }
+  return best;
+}
+
+int orc_think_value(const int16_t *w, const int16_t *bias, float scale, uint64_t P, uint64_t O, int depth) {
+  return think_ab(w, bias, scale, P, O, depth, -65, 65);
+}
+
 int orc_think(const int16_t *w, const int16_t *bias, float scale, uint64_t P, uint64_t O, int depth,
               uint64_t *optimal, int *kind) {
   const uint64_t m = orc_mobility(P, O);

@@ -69,6 +69,8 @@ void orc_best_moves_batch(const uint64_t *P, const uint64_t *O, size_t n, int de
 int  orc_think_leaf(const int16_t *w, const int16_t *bias, float scale, uint64_t P, uint64_t O);
 int  orc_think(const int16_t *w, const int16_t *bias, float scale, uint64_t P, uint64_t O, int depth,
                uint64_t *optimal, int *kind);
+/* The value alone, by fail-soft alpha-beta (fast enough for deeper tests). */
+int  orc_think_value(const int16_t *w, const int16_t *bias, float scale, uint64_t P, uint64_t O, int depth);
 void orc_think_batch(const int16_t *w, const int16_t *bias, float scale, const uint64_t *P, const uint64_t *O,
                      size_t n, int depth, int threads, int8_t *scores, uint64_t *optimal, uint8_t *kinds);
 

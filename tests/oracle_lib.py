@@ -53,6 +53,7 @@ def oracle() -> ctypes.CDLL:
                                       ctypes.c_void_p, sz, i32, i32, ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.c_void_p]
         L.orc_think_batch.restype = None
+        L.orc_think_value.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float, u64, u64, i32]
         L.orc_solve_batch.restype = ctypes.c_double
         L.orc_position_key.argtypes = [u64, u32, u64, u32]
         L.orc_position_key.restype = u64
@@ -169,6 +170,11 @@ def think(w16: np.ndarray, b16: np.ndarray, scale: float, p: int, o: int, depth:
     k = ctypes.c_int(0)
     v = oracle().orc_think(w16.ctypes.data, b16.ctypes.data, scale, p, o, depth, ctypes.byref(m), ctypes.byref(k))
     return v, m.value, k.value
+
+
+def think_value(w16: np.ndarray, b16: np.ndarray, scale: float, p: int, o: int, depth: int) -> int:
+    """Depth-limited value by alpha-beta (equal to think()'s value)."""
+    return oracle().orc_think_value(w16.ctypes.data, b16.ctypes.data, scale, p, o, depth)
 
 
 def think_leaf(w16: np.ndarray, b16: np.ndarray, scale: float, p: int, o: int) -> int:

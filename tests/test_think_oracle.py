@@ -120,3 +120,11 @@ def test_planted_sign_error_is_caught(ev):
                 best = max(best, -max(inner))
         wrong += best != v
     assert wrong > 0
+
+
+def test_alpha_beta_value_equals_the_plain_recursion(ev):
+    w, b, s = ev
+    for depth, n, e in ((1, 100, 40), (2, 100, 36), (3, 80, 30), (4, 40, 24), (5, 20, 20), (9, 40, 8), (12, 20, 10)):
+        for p, o in ol.generate(48 + depth, 9, n, e):
+            p, o = int(p), int(o)
+            assert ol.think_value(w, b, s, p, o, depth) == ol.think(w, b, s, p, o, depth)[0]
