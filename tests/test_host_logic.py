@@ -39,3 +39,17 @@ def test_cli_reads_every_config_field():
     fields = {name for name, _ in go2.Config._fields_} - {"reserved", "think_depth"}
     assert fields <= names, fields - names
     assert "think_depth" not in names
+
+
+def test_missing_library_fails_loudly():
+    # No CPU fallback: without the built HIP library every entry point raises.
+    import subprocess
+    import sys
+    code = ("import gpuothello2_amd as go2, numpy as np\n"
+            "go2.LIB_PATH = '/nonexistent/libgo2.so'\n"
+            "try:\n"
+            "    go2.solve(go2.as_boards(np.zeros((1, 2), dtype=np.uint64)), 10)\n"
+            "except ImportError as e:\n"
+            "    print('raised', e)\n")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, cwd=ROOT, timeout=120)
+    assert r.returncode == 0 and "raised" in r.stdout and "not built" in r.stdout, r.stderr
