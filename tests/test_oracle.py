@@ -89,7 +89,7 @@ def test_next_bit_order(O):
     assert O.orc_next_bit(0x30) == 0x10
 
 
-@pytest.mark.skipif(not ol.ref_available(), reason="reference host build absent (GPU box)")
+@pytest.mark.skipif(not ol.ref_available(), reason="reference host build (oracle/_ref) not present")
 def test_iteration_order_matches_reference(O):
     import ctypes
     R = ol.reference()

@@ -86,7 +86,7 @@ def test_index_equals_reference_rule(layout):
             assert t3(a) + 2 * t3(b) == feat[i] - off[k]
 
 
-@pytest.mark.skipif(not ol.ref_available(), reason="reference host build absent (GPU box)")
+@pytest.mark.skipif(not ol.ref_available(), reason="reference host build (oracle/_ref) not present")
 def test_pext_matches_reference():
     L, R = ol.oracle(), ol.reference()
     rng = np.random.default_rng(1)
