@@ -13,11 +13,13 @@
 //     "total nodes: ..." (main.cu:129,144,158-162,172);
 //   * usage text and exit code 1 for fewer than three arguments (main.cu:176-179).
 // Undefined behaviour of the reference becomes a defined error (exit code 2):
-// unreadable files, a non-numeric or out-of-range DEPTH, a board with more
-// empties than DEPTH, malformed characters, overlapping discs, device errors.
-// Five arguments select think mode (main.cu:180-183): a depth-limited search
-// of THINK_DEPTH moves scored by the evaluator in GO2_WEIGHTS, each output
-// line "<base81> <score> <move>" (main.cu:114-117).
+// unreadable files, a non-numeric DEPTH or one outside 1..60, a board with
+// more empties than DEPTH (or, solving exactly, than GO2_MAX_EMPTIES),
+// malformed characters, overlapping discs, device errors.
+// Exactly five arguments select think mode (main.cu:180-183): a depth-limited
+// search of THINK_DEPTH moves scored by the evaluator in GO2_WEIGHTS, each
+// output line "<base81> <score> <move>" (main.cu:114-117); more arguments
+// solve exactly, as in the reference.
 // Board files are read, sorted and written by the multi-threaded routines
 // of board_io.hpp, with the reference's reading rules.
 //
@@ -105,12 +107,17 @@ void apply_json_config(const char* path, go2_config* c) {
 }
 
 // Column letter and row digit (bit 0 = A1); "ps" for a pass, "--" when the
-// game is over.
+// game is over (exact solve's GO2_BEST_MOVE column).
 std::string move_text(int m) {
   if (m == GO2_MOVE_PASS) return "ps";
   if (m < 0 || m > 63) return "--";
   return std::string(1, (char)('A' + m % 8)) + (char)('1' + m / 8);
 }
+
+// Think mode writes moves as the reference's hand_to_s (main.cu:31-39): a
+// board whose game is over is passed first and reported as a pass
+// (thinker.cu:151-154,166-169), so it prints "ps" as well.
+std::string think_move_text(int m) { return m == GO2_MOVE_NONE ? "ps" : move_text(m); }
 
 int env_int(const char* name, int def) {
   const char* s = getenv(name);
@@ -124,15 +131,14 @@ int main(int argc, char** argv) {
     fprintf(stderr, "usage: %s INPUT OUTPUT DEPTH [THINK_DEPTH]\n", argv[0]);
     return 1;
   }
-  // Five arguments select think mode (reference main.cu:175-185, 69-121): a
-  // depth-limited search of THINK_DEPTH moves scored by the evaluator in
-  // GO2_WEIGHTS, each line "<board> <score> <move>".  DEPTH is read but,
-  // as in the reference, not used by think mode.
-  const bool think = argc >= 5;
+  // Exactly five arguments select think mode (reference main.cu:175-185,
+  // 69-121): a depth-limited search of THINK_DEPTH moves scored by the
+  // evaluator in GO2_WEIGHTS, each line "<board> <score> <move>".  DEPTH is
+  // read but, as in the reference, not used by think mode.
+  const bool think = argc == 5;
   char* end = nullptr;
   const long depth = strtol(argv[3], &end, 10);
-  if (!end || *end || depth < 1 || depth > (think ? 60 : GO2_MAX_EMPTIES))
-    die(2, "DEPTH must be an integer in [1, " + std::to_string(think ? 60 : GO2_MAX_EMPTIES) + "]");
+  if (!end || *end || depth < 1 || depth > 60) die(2, "DEPTH must be an integer in [1, 60]");
   long think_depth = 0;
   if (think) {
     think_depth = strtol(argv[4], &end, 10);
@@ -143,6 +149,9 @@ int main(int argc, char** argv) {
   for (size_t i = 0; i < boards.size() && !think; ++i) {
     const int e = 64 - __builtin_popcountll(boards[i].player | boards[i].opponent);
     if (e > depth) die(2, "a board has " + std::to_string(e) + " empties, more than DEPTH");
+    if (e > GO2_MAX_EMPTIES)
+      die(2, "a board has " + std::to_string(e) + " empties; the exact solver takes at most " +
+                 std::to_string(GO2_MAX_EMPTIES));
   }
   File out(fopen(argv[2], "w"));
   if (!out) die(2, std::string("cannot open output ") + argv[2]);
@@ -207,20 +216,26 @@ int main(int argc, char** argv) {
   const auto t1 = std::chrono::steady_clock::now();
   std::vector<int8_t> scores(n);
   std::vector<uint8_t> moves(with_moves ? n : 0);
-  uint64_t nodes = 0;
+  uint64_t nodes = 0, tt_stores = 0, tt_hits = 0, tt_probes = 0;
   for (size_t k = 0; k < tasks.size(); ++k) {
     go2_stats st;
     if (go2_task_download(tasks[k], scores.data() + k * kBatch, &st) != GO2_OK) die(2, go2_last_error());
     if (with_moves && go2_task_download_moves(tasks[k], moves.data() + k * kBatch) != GO2_OK)
       die(2, go2_last_error());
     nodes += st.nodes;
+    tt_stores += st.tt_stores;
+    tt_hits += st.tt_hits;
+    tt_probes += st.tt_probes;
     go2_task_destroy(tasks[k]);
   }
   const long ms = (long)std::chrono::duration_cast<std::chrono::milliseconds>(t1 - t0).count();
+  // The table's update / hit / lookup counts (main.cu:158-162, table.cuh:26-40).
   fprintf(stderr,
-          "hipSuccess (no error), elapsed: %ldms, table update count: %d, table hit: %d, table find: %d\n",
-          ms, 0, 0, 0);
-  go2cli::write_results(out.get(), boards, scores.data(), with_moves ? moves.data() : nullptr, move_text);
+          "hipSuccess (no error), elapsed: %ldms, table update count: %" PRIu64 ", table hit: %" PRIu64
+          ", table find: %" PRIu64 "\n",
+          ms, tt_stores, tt_hits, tt_probes);
+  go2cli::write_results(out.get(), boards, scores.data(), with_moves ? moves.data() : nullptr,
+                        think ? think_move_text : move_text);
   fprintf(stderr, "total nodes: %" PRIu64 "\n", nodes);
   return 0;
 }

@@ -79,13 +79,22 @@ struct IoBufs {
   }
 };
 
-// Largest number of empties a board may have: GO2_MAX_EMPTIES for the exact
-// solve, any position of a game (60) in think mode.
-static int check_depth(int max_depth, const go2_config& c) {
-  const int hi = c.think_depth > 0 ? 60 : GO2_MAX_EMPTIES;
-  if (max_depth < 1 || max_depth > hi)
-    return fail(GO2_E_ARG, "max_depth must be in [1, " + std::to_string(hi) + "]");
+// DEPTH, the reference's maximum number of empties (main.cu:127): any value
+// a game can have.  Boards with more empties than DEPTH, or (exact solve)
+// than GO2_MAX_EMPTIES, are rejected per board by the kernel.
+static int check_depth(int max_depth) {
+  if (max_depth < 1 || max_depth > 60) return fail(GO2_E_ARG, "max_depth must be in [1, 60]");
   return GO2_OK;
+}
+
+// Largest number of empties among host boards.
+static int max_empties(const go2_board* b, size_t n) {
+  int m = 0;
+  for (size_t i = 0; i < n; ++i) {
+    const int e = 64 - __builtin_popcountll(b[i].player | b[i].opponent);
+    if (e > m) m = e;
+  }
+  return m;
 }
 
 struct Events {
@@ -110,6 +119,7 @@ struct go2_task {
   size_t n = 0;
   size_t count = 0;
   int max_depth = 0;
+  int upload_max_e = 0;  // most empties among the uploaded boards: sizes the stack
   go2_config cfg{};
   hipStream_t stream = nullptr;
   Workspace ws;
@@ -191,7 +201,7 @@ int go2_solve_device_moves(const go2_board* d_boards, size_t n, int max_depth, i
   if (n && (!d_boards || !d_scores)) return fail(GO2_E_ARG, "null device array");
   go2_config c;
   if ((rc = resolve_config(cfg, &c)) != GO2_OK) return rc;
-  if ((rc = check_depth(max_depth, c)) != GO2_OK) return rc;
+  if ((rc = check_depth(max_depth)) != GO2_OK) return rc;
   hipStream_t s = (hipStream_t)hip_stream;
   if (n == 0) {
     if (stats) *stats = go2_stats{};
@@ -226,7 +236,7 @@ go2_task* go2_task_create(size_t n, int max_depth, const go2_config* cfg) {
   go2_task* t = new go2_task;
   t->n = n;
   t->max_depth = max_depth;
-  if (resolve_config(cfg, &t->cfg) != GO2_OK || check_depth(max_depth, t->cfg) != GO2_OK) {
+  if (resolve_config(cfg, &t->cfg) != GO2_OK || check_depth(max_depth) != GO2_OK) {
     delete t;
     return nullptr;
   }
@@ -255,6 +265,7 @@ int go2_task_upload(go2_task* t, const go2_board* boards, size_t count) {
     if ((e = hipStreamSynchronize(t->stream)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
   }
   t->count = count;
+  t->upload_max_e = max_empties(boards, count);
   t->launched = false;
   return GO2_OK;
 }
@@ -262,8 +273,11 @@ int go2_task_upload(go2_task* t, const go2_board* boards, size_t count) {
 int go2_task_launch(go2_task* t) {
   if (!t) return fail(GO2_E_ARG, "null task");
   int rc = GO2_OK;
+  // The stack covers the deepest uploaded board (or DEPTH, if that is
+  // smaller: deeper boards are then rejected).
+  const int depth = t->upload_max_e < t->max_depth ? (t->upload_max_e > 1 ? t->upload_max_e : 1) : t->max_depth;
   if (t->count)
-    rc = launch_solve(t->bufs.boards, t->count, t->max_depth, t->bufs.scores, t->bufs.moves, t->cfg, t->ws,
+    rc = launch_solve(t->bufs.boards, t->count, depth, t->bufs.scores, t->bufs.moves, t->cfg, t->ws,
                       t->stream, t->ev.a, t->ev.b, t->cfg.graph_launch ? &t->graph : nullptr);
   if (rc == GO2_OK) t->launched = true;
   return rc;

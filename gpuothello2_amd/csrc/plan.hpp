@@ -1,0 +1,121 @@
+// Launch plan of one solve: stack levels, LDS bytes, grid, boards per
+// wavefront and the help levels, computed on the host from the batch size,
+// DEPTH, the configuration and the device's shape.  Plain C++ with no HIP
+// dependency, so a CPU harness (tests/plan_check.cpp) checks the same code
+// the library runs.
+//
+// Counterpart in the reference: the launch arithmetic of BatchedTask
+// (solver.cu:313-333: grid ceil(n/2048) x 64 threads, DEPTH-9 upper frames,
+// 10 shared-memory levels), redone for a resident persistent grid.
+#pragma once
+#include <cstddef>
+
+#include "../../include/go2.h"
+
+namespace go2 {
+
+// Lowest stack level: nodes with three empties solve their children in
+// straight-line code, so only nodes with four or more push frames.
+constexpr int kFrameBase = 4;
+constexpr size_t kLastTableBytes = 2048;   // last-square flip counts (LDS)
+constexpr size_t kBase3TableBytes = 2048;  // base-3 table of the evaluator (LDS)
+constexpr int kDefaultLdsLevels = 6;
+constexpr size_t kFrameBytesPerLevel = 32;  // fb, mv, od (u64) + wd, sx (u32) per lane
+constexpr unsigned int kHelpSlots = 4096;
+constexpr size_t kMaxHelpLanes = size_t(1) << 19;  // slot tags hold global lane ids below 2^19
+constexpr int kExtRec = 4;  // siblings of one node out with other wavefronts at once
+
+struct StackPlan {
+  bool think = false;
+  int depth = 0;       // boards with more empties are rejected
+  int maxe = 0;        // highest stack level
+  int levels = 0;      // frame levels kFrameBase..maxe
+  int lds_levels = 0;  // of which in LDS (the lowest ones)
+  size_t lds_bytes = 0;
+};
+
+// max_depth is the caller's DEPTH (1..60).  The exact solve handles boards
+// with up to GO2_MAX_EMPTIES empties (one bit per level in 32-bit masks);
+// its stack is sized for min(DEPTH, GO2_MAX_EMPTIES).  Think mode searches
+// think_depth moves from any position (stack levels relative to each
+// board's horizon).
+inline StackPlan plan_stack(int max_depth, const go2_config& cfg) {
+  StackPlan s;
+  s.think = cfg.think_depth > 0;
+  if (s.think) {
+    s.depth = 60;
+    s.maxe = cfg.think_depth + 2;
+  } else {
+    s.depth = max_depth < GO2_MAX_EMPTIES ? max_depth : GO2_MAX_EMPTIES;
+    s.maxe = s.depth < 3 ? 3 : s.depth;
+  }
+  s.levels = s.maxe + 1 - kFrameBase;
+  if (s.levels < 1) s.levels = 1;
+  s.lds_levels = cfg.lds_levels > 0 ? cfg.lds_levels : kDefaultLdsLevels;
+  if (s.lds_levels > s.levels) s.lds_levels = s.levels;
+  s.lds_bytes = (size_t)s.lds_levels * (size_t)cfg.block_size * kFrameBytesPerLevel + kLastTableBytes +
+                (cfg.ordering == GO2_ORDER_LEARNED || s.think ? kBase3TableBytes : 0);
+  return s;
+}
+
+struct GridPlan {
+  size_t grid = 0;         // workgroups
+  size_t waves = 0;        // wavefronts
+  size_t total_lanes = 0;  // grid * block size
+  size_t g_levels = 0;     // stack levels in HBM
+  int K = 1;               // boards one wavefront owns at once
+  int split_min = 0;       // helpers take siblings at nodes with >= this many empties
+  int ext_min = 3;         // cross-wavefront export levels (when help is on)
+  int ext_lo = 3;
+  bool help = false;       // cross-wavefront help on
+};
+
+// cus: compute units of the device; per_cu: workgroups of this kernel that
+// stay resident on one CU (from the occupancy query, >= 1).
+inline GridPlan plan_grid(const StackPlan& s, size_t n, const go2_config& cfg, int cus, int per_cu) {
+  GridPlan g;
+  if (cfg.blocks_per_cu > 0 && cfg.blocks_per_cu < per_cu) per_cu = cfg.blocks_per_cu;
+  const size_t bs = (size_t)cfg.block_size;
+  const size_t waves_per_block = bs / 64;
+  g.grid = (size_t)cus * (size_t)per_cu;
+  int split_min = cfg.split_min_empties > 0 ? cfg.split_min_empties : 8;
+  if (split_min < 5) split_min = 5;
+  if (split_min > s.maxe + 1) split_min = s.maxe + 1;  // no split levels at all
+  if (s.think) split_min = 64;                         // one lane per board
+  g.split_min = split_min;
+  // Export levels: ext_min (default 12) while few wavefronts are idle, down
+  // to ext_lo (default 9) when nearly all are; never below split_min.  No
+  // node of this stack has more than maxe empties, so every export level
+  // above maxe is the same as maxe + 1 (nothing to export): capping there
+  // keeps the kernel's level shifts (level - 3) below 32 and every frame and
+  // mailbox index in range.
+  int ext_min = cfg.export_min_empties > 0
+                    ? (cfg.export_min_empties > split_min ? cfg.export_min_empties : split_min)
+                    : (split_min > 12 ? split_min : 12);
+  if (ext_min > s.maxe + 1) ext_min = s.maxe + 1;
+  int lo = cfg.export_low_empties > 0 ? cfg.export_low_empties : 9;
+  if (lo < split_min) lo = split_min;
+  const int ext_lo = lo < ext_min ? lo : ext_min;
+  g.help = !s.think && cfg.export_min_empties <= 60 && ext_lo <= s.maxe &&
+           (size_t)cus * (size_t)per_cu * bs < kMaxHelpLanes;
+  if (g.help) {
+    g.ext_min = ext_min;
+    g.ext_lo = ext_lo;
+  }
+  // Without cross-wavefront help, never launch more wavefronts than boards;
+  // with it, the whole resident grid runs and idle wavefronts take siblings.
+  const size_t max_grid = (n + waves_per_block - 1) / waves_per_block;
+  if (!g.help && g.grid > max_grid) g.grid = max_grid;
+  if (g.grid < 1) g.grid = 1;
+  g.waves = g.grid * waves_per_block;
+  int K = cfg.positions_per_wave;
+  if (K <= 0) K = (int)((n + g.waves - 1) / g.waves);
+  if (K > 64) K = 64;
+  if (K < 1) K = 1;
+  g.K = K;
+  g.total_lanes = g.grid * bs;
+  g.g_levels = (size_t)(s.levels - s.lds_levels);
+  return g;
+}
+
+}  // namespace go2

@@ -61,6 +61,7 @@
 #include "bitboard.hpp"
 #include "internal.hpp"
 #include "pattern.hpp"
+#include "plan.hpp"
 
 namespace go2 {
 
@@ -165,9 +166,6 @@ __device__ __forceinline__ int child_type(int type, int done) {
 }
 
 constexpr int kNoValue = -127;  // "no helper reported yet"
-// Lowest stack level: nodes with three empties solve their children in
-// straight-line code, so only nodes with four or more push frames.
-constexpr int kFrameBase = 4;
 
 // A child value on its way into a node, with the move that produced it:
 // ((value + 64) << 1 | exact) << 6 | square, where exact says the value beat
@@ -318,7 +316,6 @@ __device__ __forceinline__ void st_agent(T* p, T v) {
 // owner can take them back).
 __device__ __forceinline__ int sx_ext(uint32_t x) { return (int)(x >> 29); }
 constexpr uint32_t kSxExtOne = 1u << 29;
-constexpr int kExtRec = 4;
 
 // Board of the parent of (P, O), from the frame word and flip set the parent
 // stored when it moved to (P, O).
@@ -340,7 +337,7 @@ __device__ __forceinline__ void undo_move(uint64_t& P, uint64_t& O, uint64_t fb,
 // by the evaluator (rounded to whole discs); passes cost no depth.  One lane
 // per board (no helpers, no table: stored values would depend on each
 // board's horizon), learned ordering.
-template <int MAXE, int ORDER, bool THINK = false>
+template <int ORDER, bool THINK = false>
 __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
   static_assert(!THINK || ORDER == 2, "think mode needs the evaluator's tables");
   extern __shared__ uint64_t lds[];
@@ -1221,17 +1218,17 @@ __global__ void reset_counters_kernel(SolveCounters* c) {
 }
 
 // ---------------------------------------------------------------------------
-// Host side: pick the instantiation whose stack covers DEPTH, size the grid
-// to what stays resident, lay out the HBM frames, launch.
+// Host side: pick the instantiation for the ordering, size the grid to what
+// stays resident (plan.hpp), lay out the HBM frames, launch.
 
 using KernelFn = void (*)(SolveArgs);
 
-template <int MAXE>
-static KernelFn pick_order(int ordering) {
+static KernelFn pick_kernel(int ordering, bool think) {
+  if (think) return solve_kernel<2, true>;
   switch (ordering) {
-    case GO2_ORDER_STATIC: return solve_kernel<MAXE, 0>;
-    case GO2_ORDER_MOBILITY: return solve_kernel<MAXE, 1>;
-    default: return solve_kernel<MAXE, 2>;
+    case GO2_ORDER_STATIC: return solve_kernel<0>;
+    case GO2_ORDER_MOBILITY: return solve_kernel<1>;
+    default: return solve_kernel<2>;
   }
 }
 
@@ -1269,16 +1266,6 @@ extern "C" int go2_set_evaluator(const int16_t* w, const int16_t* bias, float sc
 }
 
 namespace go2 {
-
-static int stack_levels_for(int max_depth, KernelFn* fn, int ordering) {
-  const int d = max_depth < 3 ? 3 : max_depth;
-  if (d <= 12) { *fn = pick_order<12>(ordering); return 12; }
-  if (d <= 16) { *fn = pick_order<16>(ordering); return 16; }
-  if (d <= 20) { *fn = pick_order<20>(ordering); return 20; }
-  if (d <= 24) { *fn = pick_order<24>(ordering); return 24; }
-  *fn = pick_order<32>(ordering);
-  return 32;
-}
 
 Workspace::~Workspace() {
   if (ctr) (void)hipFree(ctr);
@@ -1385,21 +1372,10 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
                  const go2_config& cfg, Workspace& ws, hipStream_t stream, hipEvent_t ev0,
                  hipEvent_t ev1, GraphCache* graph) {
   if (n > 0xFFFFFF00ULL) return fail(GO2_E_ARG, "too many boards in one solve (max 2^32-256)");
-  KernelFn fn = nullptr;
-  const bool think = cfg.think_depth > 0;
-  // Think mode pushes frames for levels 4..think_depth+2 (relative to each
-  // board's horizon); the exact solve for levels 4..maxe.
-  const int maxe = think ? cfg.think_depth + 2 : stack_levels_for(max_depth, &fn, cfg.ordering);
-  if (think) fn = solve_kernel<32, 2, true>;
+  const StackPlan sp = plan_stack(max_depth, cfg);
+  const KernelFn fn = pick_kernel(cfg.ordering, sp.think);
   const int bs = cfg.block_size;
-  const int levels = max(1, maxe + 1 - kFrameBase);
-  // Keep the stack in LDS up to 9 empties: three 256-lane blocks per CU (the
-  // kernel's register budget allows three wavefronts per SIMD) with the
-  // last-square and base-3 tables; nodes nearer the root use HBM frames.
-  int lds_levels = cfg.lds_levels > 0 ? cfg.lds_levels : 6;
-  if (lds_levels > levels) lds_levels = levels;
-  const size_t lds = (size_t)lds_levels * bs * 32 + 2048 +
-                     (cfg.ordering == GO2_ORDER_LEARNED || think ? kBase3Size * 2 : 0);
+  const size_t lds = sp.lds_bytes;
   hipError_t e;
   if (lds > 65536) {
     e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -1413,35 +1389,12 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, bs, lds)) != hipSuccess)
     return hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
   if (per_cu < 1) return fail(GO2_E_ARG, "block size / LDS levels do not fit one CU");
-  if (cfg.blocks_per_cu > 0 && cfg.blocks_per_cu < per_cu) per_cu = cfg.blocks_per_cu;
-  const size_t waves_per_block = (size_t)bs / 64;
-  size_t grid = (size_t)cus * per_cu;
-  // Export levels: ext_min (default 12) while few wavefronts are idle, down
-  // to ext_lo (default 9) when nearly all are; never below split_min.
-  int split_min = cfg.split_min_empties > 0 ? cfg.split_min_empties : 8;
-  if (split_min < 5) split_min = 5;
-  if (split_min > maxe + 1) split_min = maxe + 1;  // no split levels at all
-  if (think) split_min = 64;                       // one lane per board
-  const int ext_min = cfg.export_min_empties > 0 ? max(cfg.export_min_empties, split_min) : max(12, split_min);
-  const int ext_lo = min(ext_min, max(split_min, cfg.export_low_empties > 0 ? cfg.export_low_empties : 9));
-  // Without cross-wavefront help, never launch more wavefronts than boards;
-  // with it, the whole resident grid runs and idle wavefronts take siblings.
-  // Slot tags hold global lane ids below 2^19.
-  const bool help = !think && cfg.export_min_empties <= 60 && ext_lo <= maxe &&
-                    (size_t)cus * per_cu * bs < (size_t(1) << 19);
-  const size_t max_grid = (n + waves_per_block - 1) / waves_per_block;
-  if (!help && grid > max_grid) grid = max_grid;
-  if (grid < 1) grid = 1;
-  const size_t waves = grid * waves_per_block;
-  int K = cfg.positions_per_wave;
-  if (K <= 0) K = (int)((n + waves - 1) / waves);
-  if (K > 64) K = 64;
-  if (K < 1) K = 1;
+  const GridPlan gp = plan_grid(sp, n, cfg, cus, per_cu);
+  const size_t grid = gp.grid, waves = gp.waves, total_lanes = gp.total_lanes, g_levels = gp.g_levels;
+  const int lds_levels = sp.lds_levels;
 
-  const size_t total_lanes = grid * bs;
-  const size_t g_levels = (size_t)(levels - lds_levels);
   // layout: fb, mv, od (u64) | wd, sx (u32), each [level][total_lanes]
-  const size_t bytes = total_lanes * g_levels * 32 + 5 * 256;
+  const size_t bytes = total_lanes * g_levels * kFrameBytesPerLevel + 5 * 256;
   int rc = ws.ensure(bytes);
   if (rc != GO2_OK) return rc;
   uint8_t* base = reinterpret_cast<uint8_t*>(ws.frames);
@@ -1450,27 +1403,27 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   args.scores = d_scores;
   args.moves = d_moves;
   args.n = (unsigned int)n;
-  args.depth = think ? 60 : max_depth;
+  args.depth = sp.depth;
   args.think_depth = cfg.think_depth;
   args.kff = cfg.mobility_min_empties;
-  args.split_min = split_min;
+  args.split_min = gp.split_min;
   args.cut_after = cfg.split_cut_after > 0 ? cfg.split_cut_after : 3;
   args.lds_levels = lds_levels;
-  args.K = K;
+  args.K = gp.K;
   args.total_lanes = (unsigned int)total_lanes;
   args.limit_ticks = 0;
   args.max_pops = cfg.max_pops_per_trip > 0 ? cfg.max_pops_per_trip : 2;
-  if (help) {
-    const unsigned int nslots = 4096;
-    const size_t mlev = (size_t)(maxe + 1 - ext_lo);
+  if (gp.help) {
+    const unsigned int nslots = kHelpSlots;
+    const size_t mlev = (size_t)(sp.maxe + 1 - gp.ext_lo);
     const size_t mb = ((mlev * total_lanes * 4) + 255) & ~(size_t)255;
     const size_t need = (size_t)nslots * sizeof(HelpSlot) + (3 + kExtRec) * mb;
     if ((rc = ws.ensure_help(need, stream)) != GO2_OK) return rc;
     uint8_t* hb = reinterpret_cast<uint8_t*>(ws.help);
     args.slots = reinterpret_cast<HelpSlot*>(hb);
     args.nslots = nslots;
-    args.ext_min = ext_min;
-    args.ext_lo = ext_lo;
+    args.ext_min = gp.ext_min;
+    args.ext_lo = gp.ext_lo;
     size_t o = (size_t)nslots * sizeof(HelpSlot);
     args.mb_best = reinterpret_cast<int*>(hb + o);
     o += mb;
@@ -1482,19 +1435,19 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   } else {
     args.ext_min = args.ext_lo = 3;
   }
-  if (cfg.tt_log2_buckets > 0 && !think) {
+  if (cfg.tt_log2_buckets > 0 && !sp.think) {
     if ((rc = ws.ensure_tt(cfg.tt_log2_buckets, stream)) != GO2_OK) return rc;
     if ((rc = zobrist_table(dev, &args.zob)) != GO2_OK) return rc;
     args.tt = reinterpret_cast<ulonglong2*>(ws.tt);
     args.tt_mask = (1ULL << cfg.tt_log2_buckets) - 1;
     args.tt_gen = &ws.ctr->tt_gen;
-    args.tt_min = max(max(cfg.tt_min_empties > 0 ? cfg.tt_min_empties : 12, split_min), 4);
+    args.tt_min = max(max(cfg.tt_min_empties > 0 ? cfg.tt_min_empties : 12, gp.split_min), 4);
   }
-  if (cfg.ordering == GO2_ORDER_LEARNED || think) {
+  if (cfg.ordering == GO2_ORDER_LEARNED || sp.think) {
     const EvalTables& t = g_eval[dev];
     if (!t.w)
-      return fail(GO2_E_STATE, think ? "think mode needs go2_set_evaluator() on this device"
-                                     : "GO2_ORDER_LEARNED needs go2_set_evaluator() on this device");
+      return fail(GO2_E_STATE, sp.think ? "think mode needs go2_set_evaluator() on this device"
+                                        : "GO2_ORDER_LEARNED needs go2_set_evaluator() on this device");
     args.ev_scale = t.scale;
     args.ev_w = t.w;
     args.ev_bias = t.bias;
@@ -1599,7 +1552,8 @@ int collect_stats(Workspace& ws, hipStream_t stream, hipEvent_t ev0, hipEvent_t 
   if (h.error == kTimedOut) return fail(GO2_E_TIMEOUT, "solve exceeded its time limit");
   if (h.error) {
     return fail(GO2_E_BOARD, "board " + std::to_string(h.error - 1) +
-                                 " is invalid (overlapping discs or more empties than DEPTH)");
+                                 " is invalid (overlapping discs, or more empties than DEPTH or than " +
+                                 std::to_string(GO2_MAX_EMPTIES) + ")");
   }
   if (stats) {
     *stats = go2_stats{};

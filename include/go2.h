@@ -25,14 +25,16 @@ extern "C" {
 
 #define GO2_OK 0
 #define GO2_E_ARG -1     /* bad argument (null pointer, size, depth) */
-#define GO2_E_BOARD -2   /* invalid board: overlapping discs, or more empties than DEPTH */
+#define GO2_E_BOARD -2   /* invalid board: overlapping discs, or more empties than DEPTH
+                            (or, in the exact solve, than GO2_MAX_EMPTIES) */
 #define GO2_E_DEVICE -3  /* HIP runtime failure */
 #define GO2_E_NOMEM -4   /* allocation failure */
 #define GO2_E_STATE -5   /* call out of order (e.g. download before launch) */
 #define GO2_E_PARSE -6   /* malformed base81 text */
 #define GO2_E_TIMEOUT -7 /* a solve ran past config.time_limit_ms and was abandoned */
 
-/* Largest number of empty squares one board may have. */
+/* Largest number of empty squares one board of an exact solve may have
+ * (DEPTH itself may be anything in 1..60, as the reference's is). */
 #define GO2_MAX_EMPTIES 32
 
 /* Best-move codes besides squares 0..63 (bit 0 = A1), the reference's
@@ -129,9 +131,10 @@ void go2_board_to_text(const go2_board *board, char out[17]);
 /* ---- one-shot solve ------------------------------------------------------------ */
 /* Host arrays in, host scores out; blocks until done.  max_depth is the
  * reference's DEPTH (maximum number of empties, reference README.md:48-50,
- * main.cu:127): every board must have at most max_depth empties, and
- * max_depth <= GO2_MAX_EMPTIES.  Replaces the solve path of reference
- * main.cu:123-173 minus file I/O. */
+ * main.cu:127), 1..60: a board with more empties than max_depth or than
+ * GO2_MAX_EMPTIES fails the call with GO2_E_BOARD (the reference overruns
+ * its stack instead).  Replaces the solve path of reference main.cu:123-173
+ * minus file I/O. */
 int go2_solve(const go2_board *boards, size_t n, int max_depth, int8_t *scores,
               const go2_config *cfg, go2_stats *stats);
 

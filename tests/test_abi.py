@@ -55,7 +55,7 @@ def test_text_codec_rejects_garbage():
 
 def test_argument_errors_need_no_device():
     b = go2.as_boards([0x0000000810000000], [0x0000001008000000])
-    for depth in (0, 33, -5):
+    for depth in (0, 61, -5):  # DEPTH is 1..60, as a game allows
         with pytest.raises(go2.Go2Error) as e:
             go2.solve(b, depth)
         assert e.value.code == go2.GO2_E_ARG

@@ -17,7 +17,7 @@
 Every configuration prints one JSON line.  Positions come from the GPU
 generator (random legal playouts, SplitMix64 streams).  Usage:
 
-    python -m gpuothello2_amd.workloads [c1 c2 c3 c4 c5] [--scale 0.1]
+    python workloads.py [c1 c2 c3 c4 c5] [--scale 0.1]
 
 `--scale` shrinks every batch (for quick checks); the configurations above
 are scale 1.
@@ -31,8 +31,12 @@ import time
 
 import numpy as np
 
-from . import (ORDER_LEARNED, ORDER_MOBILITY, ORDER_STATIC, Task, default_config, generate)
-from . import autotune, learn
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from gpuothello2_amd import (ORDER_LEARNED, ORDER_MOBILITY, ORDER_STATIC, Task,  # noqa: E402
+                             default_config, generate)
+from gpuothello2_amd import autotune, learn  # noqa: E402
 
 
 def log(msg: str) -> None:
@@ -41,9 +45,8 @@ def log(msg: str) -> None:
 
 
 def _oracle():
-    here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-    sys.path.insert(0, os.path.join(here, "tests"))
-    import oracle_lib  # test infrastructure: the CPU checker
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib  # test infrastructure: the CPU checker and C1's CPU leg
 
     return oracle_lib
 
