@@ -1,21 +1,35 @@
-"""Benchmark: exact-solved positions/sec at 16 empties on one MI355X.
+"""Benchmark: exact-solved positions/sec at 16 empties on one MI355X,
+learned-order batched solver vs the plain batched kernel, every score checked.
 
-One step = one exact solve of the whole batch of random-playout positions
-(default 100,000 at 16 empties), boards already resident in device memory
-(a go2 task uploaded once), timed from launch to completion.  The headline
-solver (learned move ordering when a trained table is present, else
-fastest-first mobility ordering, with a shared lock-free transposition table
-of 2^29 buckets probed at nodes with >= 8 empties, 16 boards per wavefront)
-is compared with the plain batched kernel (static corner / X-square
-ordering, no table) on the same GPU, and every score is checked: headline vs
-plain on the plain sample, headline vs the CPU oracle on the CPU sample.
+One step = one exact solve of the whole batch (default 100,000 random-playout
+positions at 16 empties, seed 20261203, config id 4), boards resident in HBM
+(a go2 task uploaded once), timed from launch to completion.  The timed
+solver is the headline configuration: learned move ordering, a shared
+lock-free transposition table of 2^29 buckets probed at nodes with >= 8
+empties, 16 boards per wavefront.
 
-Prints ONE JSON line (see the keys below).  `--gpus N` with N > 1 exits:
-this solver runs on one GPU.
+Beside the timed steps, in the same run:
+  * A/B on the first 8,192 positions of the same batch, three legs timed with
+    HIP events on the solve stream:
+      plain     static corner / X-square ordering, no table   (the plain kernel's search)
+      learned   learned ordering, no table, same launch shape (differs from plain in ordering only)
+      headline  the timed configuration                       (ordering, table, boards per wavefront)
+    reported as ratio_learned_vs_plain and ratio_headline_vs_plain, with the
+    knobs each pair differs in;
+  * every score of the last timed step (all 100,000) and of every A/B leg
+    against the oracle's scores committed in tests/golden/c4_16e.npz
+    (orc_solve_ref, the restatement of the reference search), after checking
+    the boards' SHA-256 against the fixture's;
+  * the CPU baseline: the oracle's reference-method search on a bounded
+    sample of the same positions on the host cores (16 threads, plus one).
+
+Prints ONE JSON line.  `--gpus N` with N > 1 exits: this solver runs on one
+GPU.  Exit status 1 if any score differs.
 """
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -34,11 +48,39 @@ import gpuothello2_amd as go2  # noqa: E402
 ALGO_INT_OPS_PER_NODE = 446
 # MI355X: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz peak engine clock.
 PEAK_INT32_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
+FIXTURE = os.path.join(ROOT, "tests", "golden", "c4_16e.npz")
+# HBM bytes per headline launch from two rocprofv3 --pmc runs (FETCH_SIZE,
+# WRITE_SIZE, copy-calibrated); the file names the command and kernel build.
+TRAFFIC = os.path.join(ROOT, "profiles", "solve_kernel_16e_traffic.json")
 
 
 def log(msg: str) -> None:
     sys.stderr.write(msg + "\n")
     sys.stderr.flush()
+
+
+def boards_sha256(b: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(b).view("<u8").tobytes()).hexdigest()
+
+
+def run_leg(boards, depth, cfg, label):
+    """One solve of `boards` on a fresh task: (scores, stats)."""
+    t = go2.Task(len(boards), depth, cfg)
+    try:
+        t.upload(boards)
+        t.launch()
+        t.wait()
+        s, st = t.download()
+    finally:
+        t.close()
+    log(f"[bench] {label}: {len(boards)} positions, kernel {st['kernel_ms']:.1f} ms, "
+        f"{st['nodes'] / len(boards):.0f} nodes/position")
+    return s, st
+
+
+def leg_summary(n, st, knobs):
+    return {"value": n / (st["kernel_ms"] / 1e3), "kernel_ms": st["kernel_ms"],
+            "nodes_per_position": st["nodes"] / n, "knobs": knobs}
 
 
 def main() -> int:
@@ -49,13 +91,12 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=100000, help="positions per step")
     ap.add_argument("--empties", type=int, default=16)
     ap.add_argument("--seed", type=int, default=20261203)
-    ap.add_argument("--ordering", choices=["best", "mobility", "learned", "static"], default="best")
     ap.add_argument("--tt-log2", type=int, default=29,
                     help="headline solver: shared table of 2^N 16-byte buckets (0 = none; 29 = 8 GiB)")
     ap.add_argument("--tt-min", type=int, default=8, help="headline solver: probe/store nodes with >= N empties")
     ap.add_argument("--ppw", type=int, default=16, help="headline solver: boards a wavefront owns at once")
-    ap.add_argument("--plain-n", type=int, default=256, help="positions for the plain-kernel leg (0 = skip)")
-    ap.add_argument("--plain-limit-s", type=int, default=150, help="time limit of the plain leg")
+    ap.add_argument("--ab-n", type=int, default=8192, help="positions of the A/B legs (0 = skip)")
+    ap.add_argument("--ab-limit-s", type=int, default=240, help="time limit of each A/B leg")
     ap.add_argument("--cpu-sample", type=int, default=2048,
                     help="positions for the CPU baseline leg (about 15 s on 16 threads at 16 empties)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
@@ -79,27 +120,30 @@ def main() -> int:
               "(positions shard with no exchange, see the design notes).", file=sys.stderr)
         return 2
 
+    from gpuothello2_amd import learn
+    if not os.path.exists(learn.WEIGHTS_PATH):
+        raise SystemExit(f"no trained evaluator at {learn.WEIGHTS_PATH}")
+    learned_meta = learn.install()
     E, n = args.empties, args.n
-    order_name = args.ordering
-    learned_meta = None
-    if order_name in ("best", "learned"):
-        from gpuothello2_amd import learn
-        if os.path.exists(learn.WEIGHTS_PATH):
-            learned_meta = learn.install()
-            order_name = "learned"
-        elif order_name == "learned":
-            raise SystemExit(f"no trained evaluator at {learn.WEIGHTS_PATH}")
-        else:
-            order_name = "mobility"
-    ordering = {"static": go2.ORDER_STATIC, "mobility": go2.ORDER_MOBILITY, "learned": go2.ORDER_LEARNED}[order_name]
-    cfg = go2.default_config(ordering=ordering, tt_log2_buckets=args.tt_log2, tt_min_empties=args.tt_min,
-                             positions_per_wave=args.ppw)
+    head_knobs = {"ordering": "learned", "tt_log2_buckets": args.tt_log2, "tt_min_empties": args.tt_min,
+                  "positions_per_wave": args.ppw}
+    cfg = go2.default_config(ordering=go2.ORDER_LEARNED, tt_log2_buckets=args.tt_log2,
+                             tt_min_empties=args.tt_min, positions_per_wave=args.ppw)
 
     log(f"[bench] generating {n} positions at {E} empties on the GPU")
     boards = go2.generate(args.seed, 4, n, E)
+    # Oracle scores of this batch (or of its prefix), when the fixture covers it.
+    # Position i depends only on (seed, config id, i), so a shorter batch is
+    # a prefix of the fixture's and a longer one starts with it.
+    want = None
+    if os.path.exists(FIXTURE):
+        fx = np.load(FIXTURE)
+        fn = int(fx["n"])
+        if (int(fx["seed"]), int(fx["config_id"]), int(fx["empties_min"])) == (args.seed, 4, E):
+            full = boards[:fn] if n >= fn else go2.generate(args.seed, 4, fn, E)
+            if boards_sha256(full) == str(fx["boards_sha256"]):
+                want = fx["score"][:min(n, fn)]
     task = go2.Task(n, E, cfg)
-    task.upload(boards)
-
     # Warmup: the same task on a small prefix (code load, clocks, the task's
     # table and stack allocated); the timed steps then solve the whole batch.
     # Table entries carry a launch generation, so nothing a warmup stored is
@@ -111,7 +155,7 @@ def main() -> int:
         task.wait()
         log(f"[bench] warmup {i + 1}/{args.warmup} done ({wn} positions)")
     task.upload(boards)
-    kernel_ms, nodes = [], []
+    kernel_ms, nodes, step_scores = [], [], []
     t0 = time.perf_counter()
     for i in range(args.steps):
         task.launch()
@@ -119,55 +163,62 @@ def main() -> int:
         scores, st = task.download()
         kernel_ms.append(st["kernel_ms"])
         nodes.append(st["nodes"])
+        step_scores.append(scores)
         log(f"[bench] step {i + 1}/{args.steps}: kernel {st['kernel_ms']:.1f} ms, nodes {st['nodes']}")
     t1 = time.perf_counter()
-    # download() is outside the solve; subtract nothing: it copies n bytes.
+    # download() (n bytes of scores + counters) sits inside the timed loop.
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = n / (ms_per_step / 1e3)
     k_ms = float(np.mean(kernel_ms))
     nodes_per_launch = float(np.mean(nodes))
     achieved = nodes_per_launch * ALGO_INT_OPS_PER_NODE / (k_ms / 1e3) / 1e12
     task.close()
+    steps_agree = all(np.array_equal(s, step_scores[0]) for s in step_scores)
+    match = {"steps_agree": steps_agree}
+    if want is not None:
+        k = len(want)
+        bad = int(np.sum(step_scores[-1][:k] != want))
+        match["headline_vs_oracle_fixture"] = {"checked": k, "mismatches": bad}
+        log(f"[bench] headline scores vs oracle fixture: {k - bad}/{k} equal")
 
-    # ---- plain kernel on a prefix of the same positions --------------------------------
-    # Static ordering costs ~20x the nodes at 16 empties, so this leg runs on a
-    # small prefix under a time limit; the headline solver is re-timed on the
-    # same prefix (same batch size) for the ratio.
-    pn = min(args.plain_n, n)
-    plain_value = None
-    plain_bound = None
-    plain_match = None
-    head_small_value = None
-    if pn > 0:
-        log(f"[bench] plain leg: {pn} positions, static ordering")
-        plain_cfg = go2.default_config(ordering=go2.ORDER_STATIC, time_limit_ms=args.plain_limit_s * 1000)
-        pt = go2.Task(pn, E, plain_cfg)
-        pt.upload(boards[:pn])
-        try:
-            pt.launch()
-            pt.wait()
-            plain_scores, pst = pt.download()
-            plain_value = pn / (pst["kernel_ms"] / 1e3)
-        except go2.Go2Error as e:
-            log(f"[bench] plain leg did not finish: {e}")
-            plain_scores = None
-            plain_bound = pn / args.plain_limit_s  # it took longer than the limit
-        pt.close()
-        ht = go2.Task(pn, E, cfg)
-        ht.upload(boards[:pn])
-        ht.launch()
-        ht.wait()
-        head_scores, hst = ht.download()
-        ht.close()
-        head_small_value = pn / (hst["kernel_ms"] / 1e3)
-        if plain_scores is not None:
-            plain_match = bool(np.array_equal(plain_scores, head_scores) and np.array_equal(head_scores, scores[:pn]))
-            log(f"[bench] plain on {pn}: {plain_value:.1f} pos/s; headline on the same {pn}: "
-                f"{head_small_value:.1f} pos/s; scores match: {plain_match}")
+    # ---- A/B on the first ab_n positions ---------------------------------------------
+    ab = None
+    an = min(args.ab_n, n)
+    if an > 0:
+        limit = args.ab_limit_s * 1000
+        b = boards[:an]
+        legs = {
+            "plain": (go2.default_config(ordering=go2.ORDER_STATIC, time_limit_ms=limit),
+                      {"ordering": "static", "tt_log2_buckets": 0, "positions_per_wave": "auto"}),
+            "learned": (go2.default_config(ordering=go2.ORDER_LEARNED, time_limit_ms=limit),
+                        {"ordering": "learned", "tt_log2_buckets": 0, "positions_per_wave": "auto"}),
+            "headline": (go2.Config.from_buffer_copy(cfg), head_knobs),
+        }
+        ab = {"positions": an, "sample": f"first {an} of the same positions",
+              "same_launch_shape": "persistent resident grid, helper lanes and cross-wavefront help in every leg"}
+        got = {}
+        for name, (c, knobs) in legs.items():
+            c.time_limit_ms = limit
+            try:
+                s, st = run_leg(b, E, c, f"A/B {name}")
+                ab[name] = leg_summary(an, st, knobs)
+                got[name] = s
+            except go2.Go2Error as e:
+                log(f"[bench] A/B {name} did not finish: {e}")
+                ab[name] = {"value_upper_bound": an / args.ab_limit_s, "knobs": knobs, "error": str(e)}
+        if "value" in ab.get("plain", {}):
+            for name in ("learned", "headline"):
+                if "value" in ab.get(name, {}):
+                    ab[f"ratio_{name}_vs_plain"] = ab[name]["value"] / ab["plain"]["value"]
+        ab["knobs_differing"] = {"learned_vs_plain": ["ordering"],
+                                 "headline_vs_plain": ["ordering", "tt_log2_buckets", "tt_min_empties",
+                                                       "positions_per_wave"]}
+        ref = want[:an] if want is not None and len(want) >= an else step_scores[-1][:an]
+        match["ab_legs"] = {name: int(np.sum(s != ref)) for name, s in got.items()}
+        match["ab_reference"] = "oracle fixture" if want is not None and len(want) >= an else "headline step"
 
     # ---- CPU baseline: the reference method on the host cores ----------------------------
     cpu = None
-    cpu_match = None
     if not args.skip_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as ol
@@ -176,13 +227,14 @@ def main() -> int:
         k = min(args.cpu_sample, n)
         b2 = np.stack([boards["player"][:k], boards["opponent"][:k]], 1)
         cs, cnodes, secs = ol.solve_batch(b2, E, threads=threads)
-        cpu_match = bool(np.array_equal(cs, scores[:k]))
+        cpu_bad = int(np.sum(cs != step_scores[-1][:k]))
         k1 = min(args.cpu_single_sample, k)
         single = None
         if k1 > 0:
             c1, _, secs1 = ol.solve_batch(b2[:k1], E, threads=1)
-            cpu_match = cpu_match and bool(np.array_equal(c1, scores[:k1]))
+            cpu_bad += int(np.sum(c1 != step_scores[-1][:k1]))
             single = {"value": k1 / secs1, "positions": k1, "seconds": secs1}
+        match["cpu_sample_vs_gpu"] = {"checked": k, "mismatches": cpu_bad}
         model = None
         try:
             with open("/proc/cpuinfo") as f:
@@ -194,13 +246,19 @@ def main() -> int:
                "sample": f"first {k} of the same {E}-empty positions, reference method "
                          f"(fastest-first in the top DEPTH-9 plies, corner/LSB order below, no table), "
                          f"{threads} threads, {secs:.1f} s"}
-        log(f"[bench] cpu baseline {k / secs:.1f} pos/s on {threads} threads; match={cpu_match}")
+        log(f"[bench] cpu baseline {k / secs:.1f} pos/s on {threads} threads; mismatches {cpu_bad}")
 
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "solve_kernel_16e_traffic.json")
-    if os.path.exists(tpath) and n == 100000 and E == 16:
-        with open(tpath) as f:
-            traffic = json.load(f)["bytes_per_launch"]  # rocprofv3 FETCH_SIZE/WRITE_SIZE of this step
+    traffic, traffic_src = None, None
+    if os.path.exists(TRAFFIC) and n == 100000 and E == 16:
+        with open(TRAFFIC) as f:
+            tj = json.load(f)
+        if tj.get("config") == head_knobs:
+            traffic = tj["bytes_per_launch"]
+            traffic_src = tj["source"]
+    mismatches = [match[k_]["mismatches"] for k_ in ("headline_vs_oracle_fixture", "cpu_sample_vs_gpu")
+                  if k_ in match] + list(match.get("ab_legs", {}).values())
+    ok = steps_agree and sum(mismatches) == 0
+    match["all_equal"] = bool(ok)
     line = {
         "metric": "exact-solved positions/sec @16 empties, 1 GPU (learned-order vs plain), 100% match",
         "value": value,
@@ -212,33 +270,26 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int64",
+        "dtype": "u64",
         "data": "synthetic",
         "config": {
             "workload": f"exact solve of {n} random-playout positions at {E} empties",
-            "positions": n, "empties": E, "ordering": order_name,
-            "table_buckets_log2": args.tt_log2, "table_min_empties": args.tt_min,
-            "positions_per_wave": args.ppw,
-            "plain_positions": pn, "plain_value": plain_value,
-            "headline_value_same_sample": head_small_value,
-            "ratio_vs_plain_same_sample": (head_small_value / plain_value) if plain_value else None,
-            "plain_value_upper_bound": plain_bound,
-            "ratio_vs_plain_lower_bound": (head_small_value / plain_bound) if plain_bound else None,
-            "score_match_plain": plain_match, "score_match_cpu": cpu_match,
+            "positions": n, "empties": E, **head_knobs,
             "nodes_per_position": nodes_per_launch / n,
-            "table": {k: st[k] for k in ("tt_probes", "tt_hits", "tt_cuts", "tt_stores")},
-            "evaluator": None if learned_meta is None else {
-                k: learned_meta[k] for k in ("positions", "empties", "split", "valid_mse",
-                                             "valid_mse_constant_predictor")},
+            "table": {k_: st[k_] for k_ in ("tt_probes", "tt_hits", "tt_cuts", "tt_stores")},
+            "ab": ab,
+            "score_match": match,
+            "evaluator": {k_: learned_meta[k_] for k_ in ("positions", "empties", "split", "valid_mse",
+                                                          "valid_mse_constant_predictor")},
         },
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_INT32_TOPS,
                      "unit": "Tops/s (int32 lane-ops)", "frac": achieved / PEAK_INT32_TOPS,
-                     "traffic": traffic, "kernel_ms": k_ms,
+                     "traffic": traffic, "traffic_source": traffic_src, "kernel_ms": k_ms,
                      "ops_per_node": ALGO_INT_OPS_PER_NODE},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
-    return 0 if plain_match is not False and cpu_match is not False else 1
+    return 0 if ok else 1
 
 
 if __name__ == "__main__":

@@ -22,10 +22,16 @@ def boards():
     return b, want
 
 
+@pytest.fixture(scope="module")
+def evaluator():
+    from gpuothello2_amd import learn
+    return learn.install()
+
+
 @pytest.mark.parametrize("log2,tmin", [(10, 6), (16, 8), (22, 10), (24, 12)])
-def test_scores_exact_with_table(boards, log2, tmin):
+def test_scores_exact_with_table(boards, evaluator, log2, tmin):
     b, want = boards
-    for ordering in (go2.ORDER_STATIC, go2.ORDER_MOBILITY):
+    for ordering in (go2.ORDER_STATIC, go2.ORDER_MOBILITY, go2.ORDER_LEARNED):
         got, st = go2.solve(go2.as_boards(b), 14, cfg(ordering=ordering, tt_log2_buckets=log2,
                                                       tt_min_empties=tmin, split_min_empties=tmin))
         assert np.array_equal(got, want), (log2, tmin, ordering)
