@@ -46,8 +46,9 @@ EXPORTED_SYMBOLS = (
     "go2_task_wait", "go2_task_download", "go2_task_destroy", "go2_generate",
     "go2_generate_device", "go2_pattern_info", "go2_pattern_features_device",
     "go2_pattern_eval_device", "go2_sgd_gradient_device", "go2_sgd_apply_device",
-    "go2_quantize_weights_device", "go2_set_evaluator",
+    "go2_quantize_weights_device", "go2_set_evaluator", "go2_solve_device_noalloc",
 )
+NOALLOC_SCRATCH_BYTES = 128 << 20
 
 BOARD_DTYPE = np.dtype([("player", "<u8"), ("opponent", "<u8")])
 
@@ -135,6 +136,7 @@ def lib() -> ctypes.CDLL:
     L.go2_solve_moves.argtypes = [vp, sz, i32, vp, vp, ctypes.POINTER(Config), ctypes.POINTER(Stats)]
     L.go2_solve_device_moves.argtypes = [vp, sz, i32, vp, vp, ctypes.POINTER(Config), ctypes.POINTER(Stats), vp]
     L.go2_task_download_moves.argtypes = [vp, vp]
+    L.go2_solve_device_noalloc.argtypes = [vp, sz, i32, vp, vp, ctypes.POINTER(Config), ctypes.POINTER(Stats), vp]
     L.go2_think.argtypes = [vp, sz, i32, vp, vp, ctypes.POINTER(Config), ctypes.POINTER(Stats)]
     L.go2_task_create.argtypes = [sz, i32, ctypes.POINTER(Config)]
     L.go2_task_create.restype = vp
@@ -251,6 +253,19 @@ def solve_device(d_boards_ptr: int, n: int, max_depth: int, d_scores_ptr: int,
     _check(lib().go2_solve_device(d_boards_ptr, n, max_depth, d_scores_ptr,
                                   ctypes.byref(config) if config is not None else None,
                                   ctypes.byref(st), stream))
+    return st.as_dict()
+
+
+def solve_device_noalloc(d_boards_ptr: int, n: int, max_depth: int, d_scores_ptr: int,
+                         d_moves_ptr: int | None = None, config: Config | None = None,
+                         stream: int | None = None) -> dict:
+    """go2_solve_device_noalloc: boards already in device memory, no device
+    allocation by the library (its scratch is a static device array shared
+    by every call on the device); returns stats."""
+    st = Stats()
+    _check(lib().go2_solve_device_noalloc(d_boards_ptr, n, max_depth, d_scores_ptr, d_moves_ptr,
+                                          ctypes.byref(config) if config is not None else None,
+                                          ctypes.byref(st), stream))
     return st.as_dict()
 
 

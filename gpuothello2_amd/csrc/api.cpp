@@ -214,6 +214,27 @@ int go2_solve_device_moves(const go2_board* d_boards, size_t n, int max_depth, i
   return collect_stats(ws, s, ev.a, ev.b, stats);
 }
 
+int go2_solve_device_noalloc(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_scores,
+                             uint8_t* d_moves, const go2_config* cfg, go2_stats* stats, void* hip_stream) {
+  int rc;
+  if (n && (!d_boards || !d_scores)) return fail(GO2_E_ARG, "null device array");
+  go2_config c;
+  if ((rc = resolve_config(cfg, &c)) != GO2_OK) return rc;
+  if ((rc = check_depth(max_depth)) != GO2_OK) return rc;
+  if (c.tt_log2_buckets > 0)
+    return fail(GO2_E_ARG, "this entry point has no transposition table (config.tt_log2_buckets must be 0)");
+  hipStream_t s = (hipStream_t)hip_stream;
+  if (n == 0) {
+    if (stats) *stats = go2_stats{};
+    return GO2_OK;
+  }
+  Workspace ws;  // fixed mode: nothing allocated, nothing freed
+  if ((rc = noalloc_scratch(&ws.fixed, &ws.fixed_cap)) != GO2_OK) return rc;
+  if ((rc = launch_solve(d_boards, n, max_depth, d_scores, d_moves, c, ws, s, nullptr, nullptr)) != GO2_OK)
+    return rc;
+  return collect_stats(ws, s, nullptr, nullptr, stats);
+}
+
 int go2_solve(const go2_board* boards, size_t n, int max_depth, int8_t* scores,
               const go2_config* cfg, go2_stats* stats) {
   return go2_solve_moves(boards, n, max_depth, scores, nullptr, cfg, stats);

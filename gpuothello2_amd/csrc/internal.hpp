@@ -32,6 +32,10 @@ struct SolveCounters {
   unsigned int active_waves;
   unsigned int idle_waves;  // wavefronts polling for exported siblings
   unsigned long long exports;
+  // Device clock (s_memrealtime, 100 MHz) at the reset and, atomically
+  // maximised, when the last wavefront of the solve left.
+  unsigned long long t_start;
+  unsigned long long t_end;
 };
 
 constexpr unsigned int kTimedOut = 0xFFFFFFFFu;
@@ -44,8 +48,12 @@ int hip_fail(hipError_t e, const char* what);
 int resolve_config(const go2_config* cfg, go2_config* out);
 
 // Device memory of a solve that outlives one launch: counters and the
-// per-lane search stacks kept in HBM.  Grown on demand, freed on destruction.
+// per-lane search stacks kept in HBM.  Grown on demand, freed on destruction;
+// or, in fixed mode (`fixed` set), carved from caller-provided device memory
+// and never allocated or freed.
 struct Workspace {
+  uint8_t* fixed = nullptr;  // fixed mode: base of the region, fixed_cap bytes
+  size_t fixed_cap = 0;
   SolveCounters* ctr = nullptr;
   void* frames = nullptr;
   size_t frames_bytes = 0;
@@ -65,7 +73,14 @@ struct Workspace {
   int ensure(size_t frame_bytes);
   int ensure_tt(int log2_buckets, hipStream_t stream);
   int ensure_help(size_t bytes, hipStream_t stream);
+  // Fixed mode: counters, then help_bytes of help slots and mailboxes, then
+  // frame_bytes of frames, within fixed_cap.
+  int carve(size_t help_bytes, size_t frame_bytes);
 };
+
+// Device memory of the no-allocation entry point on the current device (a
+// static device array, its address looked up once per device).
+int noalloc_scratch(uint8_t** base, size_t* bytes);
 
 // A launch sequence (counter reset + solve kernel) captured in a HIP graph,
 // re-captured when any launch argument changes.
@@ -90,7 +105,8 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
                  const go2_config& cfg, Workspace& ws, hipStream_t stream, hipEvent_t ev0,
                  hipEvent_t ev1, GraphCache* graph = nullptr);
 
-// Copy counters back (stream-ordered, then synchronises the stream) and fill stats.
+// Copy counters back (stream-ordered, then synchronises the stream) and fill
+// stats; kernel_ms from the events when given, else from the device clock.
 int collect_stats(Workspace& ws, hipStream_t stream, hipEvent_t ev0, hipEvent_t ev1,
                   go2_stats* stats);
 

@@ -24,6 +24,8 @@ constexpr size_t kFrameBytesPerLevel = 32;  // fb, mv, od (u64) + wd, sx (u32) p
 constexpr unsigned int kHelpSlots = 4096;
 constexpr size_t kMaxHelpLanes = size_t(1) << 19;  // slot tags hold global lane ids below 2^19
 constexpr int kExtRec = 4;  // siblings of one node out with other wavefronts at once
+constexpr size_t kHelpSlotBytes = 64;   // one exported sibling (HelpSlot)
+constexpr size_t kCounterBytes = 256;   // the launch counters (SolveCounters), rounded up
 
 struct StackPlan {
   bool think = false;
@@ -116,6 +118,53 @@ inline GridPlan plan_grid(const StackPlan& s, size_t n, const go2_config& cfg, i
   g.total_lanes = g.grid * bs;
   g.g_levels = (size_t)(s.levels - s.lds_levels);
   return g;
+}
+
+inline size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Device bytes of the HBM stack frames: fb, mv, od (u64) and wd, sx (u32),
+// each [g_levels][total_lanes], each region 256-byte aligned.
+inline size_t frame_bytes(const GridPlan& g) {
+  const size_t cells = g.total_lanes * g.g_levels;
+  return 3 * round256(cells * 8) + 2 * round256(cells * 4);
+}
+
+// Device bytes of cross-wavefront help: the slots, then four mailboxes (best
+// value, reports, cut flag, reclaim records x kExtRec) per help level and lane.
+inline size_t mailbox_bytes(const StackPlan& s, const GridPlan& g) {
+  return round256((size_t)(s.maxe + 1 - g.ext_lo) * g.total_lanes * 4);
+}
+inline size_t help_bytes(const StackPlan& s, const GridPlan& g) {
+  if (!g.help) return 0;
+  return (size_t)kHelpSlots * kHelpSlotBytes + (3 + kExtRec) * mailbox_bytes(s, g);
+}
+
+// Everything a solve without a table keeps in device memory besides its
+// boards and results: counters, help slots and mailboxes, HBM frames.
+inline size_t scratch_bytes(const StackPlan& s, const GridPlan& g) {
+  return kCounterBytes + help_bytes(s, g) + frame_bytes(g);
+}
+
+// The grid of plan_grid, shrunk until its scratch fits `capacity` bytes:
+// fewer resident workgroups per CU first, then no cross-wavefront help.
+// False when even one workgroup per CU without help does not fit.
+inline bool plan_grid_within(const StackPlan& s, size_t n, go2_config cfg, int cus, int per_cu,
+                             size_t capacity, GridPlan* out) {
+  if (cfg.blocks_per_cu > 0 && cfg.blocks_per_cu < per_cu) per_cu = cfg.blocks_per_cu;
+  for (;;) {
+    const GridPlan g = plan_grid(s, n, cfg, cus, per_cu);
+    if (scratch_bytes(s, g) <= capacity) {
+      *out = g;
+      return true;
+    }
+    if (per_cu > 1) {
+      --per_cu;
+    } else if (g.help) {
+      cfg.export_min_empties = 61;  // > 60: no cross-wavefront help
+    } else {
+      return false;
+    }
+  }
 }
 
 }  // namespace go2

@@ -292,7 +292,7 @@ struct HelpSlot {
   unsigned int owner;  // global lane id of the owner
   unsigned int state;  // kSlotEmpty / kSlotWriting / full (tag << 2 | kSlotFull) / kSlotTaken
   int sq;              // the sibling's move square
-  unsigned int pad[7];
+  unsigned int pad[5];  // 64 bytes: one slot per cache line
 };
 constexpr unsigned int kSlotEmpty = 0, kSlotWriting = 1, kSlotFull = 2, kSlotTaken = 3;
 // A full slot carries its owner's tag (global lane id < 2^19, frame level
@@ -1195,10 +1195,15 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
   }
 
   flush_counters();
-  if (lane == 0) atomicAdd(&a.ctr->lane_slots, 64ull * trip);  // every lane takes part in every trip
+  if (lane == 0) {
+    atomicAdd(&a.ctr->lane_slots, 64ull * trip);  // every lane takes part in every trip
+    atomicMax(&a.ctr->t_end, (unsigned long long)__builtin_amdgcn_s_memrealtime());
+  }
 }
 
 __global__ void reset_counters_kernel(SolveCounters* c) {
+  c->t_start = __builtin_amdgcn_s_memrealtime();
+  c->t_end = 0;
   c->active_waves = 0;
   c->idle_waves = 0;
   c->exports = 0;
@@ -1268,10 +1273,52 @@ extern "C" int go2_set_evaluator(const int16_t* w, const int16_t* bias, float sc
 namespace go2 {
 
 Workspace::~Workspace() {
+  if (fixed) return;  // carved from memory the workspace does not own
   if (ctr) (void)hipFree(ctr);
   if (frames) (void)hipFree(frames);
   if (tt) (void)hipFree(tt);
   if (help) (void)hipFree(help);
+}
+
+static_assert(sizeof(SolveCounters) <= kCounterBytes, "counters outgrew their scratch slot");
+static_assert(sizeof(HelpSlot) == kHelpSlotBytes, "help slot size");
+
+int Workspace::carve(size_t help_b, size_t frame_b) {
+  if (!fixed) return fail(GO2_E_STATE, "workspace is not in fixed mode");
+  if (kCounterBytes + help_b + frame_b > fixed_cap) return fail(GO2_E_ARG, "solve does not fit the scratch");
+  ctr = reinterpret_cast<SolveCounters*>(fixed);
+  help = help_b ? fixed + kCounterBytes : nullptr;
+  help_bytes = help_b;
+  frames = fixed + kCounterBytes + help_b;
+  frames_bytes = frame_b;
+  return GO2_OK;
+}
+
+// ---- the no-allocation entry point's memory ------------------------------------
+// One static device array: each device the library runs on holds its own
+// copy until the process ends.  Host code never passes the array's name to
+// a kernel (on the host it is only a stand-in for the symbol); it looks the
+// device address up once per device and passes that.
+static __device__ __align__(256) uint8_t scratch[GO2_NOALLOC_SCRATCH_BYTES];
+static uint8_t* g_scratch[64];
+static std::mutex g_scratch_mutex;
+
+int noalloc_scratch(uint8_t** base, size_t* bytes) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev < 0 || dev >= 64) return fail(GO2_E_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    if ((e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch))) != hipSuccess)
+      return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (!p) return fail(GO2_E_DEVICE, "hipGetSymbolAddress(scratch) returned no address");
+    g_scratch[dev] = reinterpret_cast<uint8_t*>(p);
+  }
+  *base = g_scratch[dev];
+  *bytes = GO2_NOALLOC_SCRATCH_BYTES;
+  return GO2_OK;
 }
 
 __global__ void clear_tt_kernel(ulonglong2* tt, unsigned long long n) {
@@ -1389,14 +1436,25 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, bs, lds)) != hipSuccess)
     return hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
   if (per_cu < 1) return fail(GO2_E_ARG, "block size / LDS levels do not fit one CU");
-  const GridPlan gp = plan_grid(sp, n, cfg, cus, per_cu);
+  GridPlan gp;
+  int rc;
+  if (ws.fixed) {
+    // Memory the caller provides: no table, and a grid whose frames and help
+    // fit (fewer workgroups per CU, then no cross-wavefront help).
+    if (cfg.tt_log2_buckets > 0 && !sp.think)
+      return fail(GO2_E_ARG, "this entry point has no transposition table (config.tt_log2_buckets must be 0)");
+    if (!plan_grid_within(sp, n, cfg, cus, per_cu, ws.fixed_cap, &gp))
+      return fail(GO2_E_ARG, "DEPTH " + std::to_string(max_depth) + " needs a search stack larger than the " +
+                                 std::to_string(ws.fixed_cap >> 20) + " MiB this entry point holds");
+    if ((rc = ws.carve(help_bytes(sp, gp), frame_bytes(gp))) != GO2_OK) return rc;
+  } else {
+    gp = plan_grid(sp, n, cfg, cus, per_cu);
+    if ((rc = ws.ensure(frame_bytes(gp))) != GO2_OK) return rc;
+    if (gp.help && (rc = ws.ensure_help(help_bytes(sp, gp), stream)) != GO2_OK) return rc;
+  }
   const size_t grid = gp.grid, waves = gp.waves, total_lanes = gp.total_lanes, g_levels = gp.g_levels;
   const int lds_levels = sp.lds_levels;
-
-  // layout: fb, mv, od (u64) | wd, sx (u32), each [level][total_lanes]
-  const size_t bytes = total_lanes * g_levels * kFrameBytesPerLevel + 5 * 256;
-  int rc = ws.ensure(bytes);
-  if (rc != GO2_OK) return rc;
+  // frames: fb, mv, od (u64) | wd, sx (u32), each [level][total_lanes]
   uint8_t* base = reinterpret_cast<uint8_t*>(ws.frames);
   SolveArgs args{};
   args.boards = d_boards;
@@ -1415,10 +1473,7 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   args.max_pops = cfg.max_pops_per_trip > 0 ? cfg.max_pops_per_trip : 2;
   if (gp.help) {
     const unsigned int nslots = kHelpSlots;
-    const size_t mlev = (size_t)(sp.maxe + 1 - gp.ext_lo);
-    const size_t mb = ((mlev * total_lanes * 4) + 255) & ~(size_t)255;
-    const size_t need = (size_t)nslots * sizeof(HelpSlot) + (3 + kExtRec) * mb;
-    if ((rc = ws.ensure_help(need, stream)) != GO2_OK) return rc;
+    const size_t mb = mailbox_bytes(sp, gp);
     uint8_t* hb = reinterpret_cast<uint8_t*>(ws.help);
     args.slots = reinterpret_cast<HelpSlot*>(hb);
     args.nslots = nslots;
@@ -1470,11 +1525,7 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   args.g_od = reinterpret_cast<uint64_t*>(take(total_lanes * g_levels * 8));
   args.g_wd = reinterpret_cast<uint32_t*>(take(total_lanes * g_levels * 4));
   args.g_sx = reinterpret_cast<uint32_t*>(take(total_lanes * g_levels * 4));
-  if (off > ws.frames_bytes) {
-    rc = ws.ensure(off);
-    if (rc != GO2_OK) return rc;
-    return launch_solve(d_boards, n, max_depth, d_scores, d_moves, cfg, ws, stream, ev0, ev1, graph);
-  }
+  if (off != frame_bytes(gp)) return fail(GO2_E_STATE, "frame layout does not match its size");
   // s_memrealtime runs at 100 MHz; each wavefront adds its own start time.
   if (cfg.time_limit_ms > 0) args.limit_ticks = (unsigned long long)cfg.time_limit_ms * 100000ULL;
   ws.grid_waves = waves;
@@ -1570,7 +1621,11 @@ int collect_stats(Workspace& ws, hipStream_t stream, hipEvent_t ev0, hipEvent_t 
     stats->exports = h.exports;
     stats->grid_waves = ws.grid_waves;
     float ms = 0.f;
-    if (ev0 && ev1 && hipEventElapsedTime(&ms, ev0, ev1) == hipSuccess) stats->kernel_ms = ms;
+    if (ev0 && ev1) {
+      if (hipEventElapsedTime(&ms, ev0, ev1) == hipSuccess) stats->kernel_ms = ms;
+    } else if (h.t_end > h.t_start) {
+      stats->kernel_ms = (double)(h.t_end - h.t_start) * 1e-5;  // 100 MHz ticks
+    }
   }
   return GO2_OK;
 }

@@ -155,6 +155,26 @@ int go2_solve_device_moves(const go2_board *d_boards, size_t n, int max_depth, i
                            uint8_t *d_moves, const go2_config *cfg, go2_stats *stats,
                            void *hip_stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): go2_solve_device_moves without any allocation.  The
+ * solve's counters, help slots, mailboxes and HBM stack frames live in one
+ * static device array of GO2_NOALLOC_SCRATCH_BYTES that each device the
+ * library runs on holds until the process ends; the caller passes boards,
+ * scores and moves (may be NULL) in device memory.  The scratch does not grow
+ * with n; when a DEPTH's stack does not fit it at full occupancy, the grid
+ * shrinks (fewer workgroups per CU, then no cross-wavefront help), and a
+ * DEPTH no grid fits fails with GO2_E_ARG.  No transposition table
+ * (config->tt_log2_buckets must be 0); stats->kernel_ms comes from the device
+ * clock.  Every call on one device shares that device's array and leaves its
+ * contents behind (each call resets what must start at zero): calls that
+ * overlap on one GPU race, so serialise them (one stream, or a lock).
+ * Waits for the solve like go2_solve_device_moves. */
+#define GO2_NOALLOC_SCRATCH_BYTES (128u << 20)
+int go2_solve_device_noalloc(const go2_board *d_boards, size_t n, int max_depth, int8_t *d_scores,
+                             uint8_t *d_moves, const go2_config *cfg, go2_stats *stats,
+                             void *hip_stream);
+
 /* Think mode (replaces BatchedThinkTask, reference thinker.cu:351-389 and
  * main.cu:69-121): depth-limited negamax over `depth` (1..30) moves from each
  * board (passes cost no depth; a game that ends first scores exactly), the

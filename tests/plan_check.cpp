@@ -1,7 +1,8 @@
 // CPU harness over the library's launch plan (gpuothello2_amd/csrc/plan.hpp):
-//   plan_check N MAX_DEPTH CUS PER_CU [field=value ...]
+//   plan_check N MAX_DEPTH CUS PER_CU [field=value ...] [capacity=BYTES]
 // fills a go2_config with go2_config_default's values, applies the given
-// integer fields, and prints the plan as "key value" lines.  Built and run
+// integer fields, and prints the plan as "key value" lines; with capacity=,
+// the plan shrunk to fit that many bytes of scratch (plan_grid_within).  Built and run
 // by tests/test_launch_plan.py with g++ (no HIP needed).
 #include <cstdio>
 #include <cstdlib>
@@ -38,9 +39,14 @@ int main(int argc, char** argv) {
                 {"export_min_empties", &c.export_min_empties},
                 {"export_low_empties", &c.export_low_empties},
                 {"think_depth", &c.think_depth}};
+  long long capacity = -1;
   for (int i = 5; i < argc; ++i) {
     const char* eq = std::strchr(argv[i], '=');
     if (!eq) return 2;
+    if (!std::strncmp(argv[i], "capacity=", 9)) {
+      capacity = std::atoll(eq + 1);
+      continue;
+    }
     bool known = false;
     for (auto& f : fields)
       if (std::strlen(f.name) == (size_t)(eq - argv[i]) && !std::strncmp(f.name, argv[i], eq - argv[i])) {
@@ -50,7 +56,17 @@ int main(int argc, char** argv) {
     if (!known) return 2;
   }
   const go2::StackPlan s = go2::plan_stack(std::atoi(argv[2]), c);
-  const go2::GridPlan g = go2::plan_grid(s, (size_t)std::atoll(argv[1]), c, std::atoi(argv[3]), std::atoi(argv[4]));
+  go2::GridPlan g;
+  if (capacity >= 0) {
+    const bool fits = go2::plan_grid_within(s, (size_t)std::atoll(argv[1]), c, std::atoi(argv[3]),
+                                            std::atoi(argv[4]), (size_t)capacity, &g);
+    std::printf("fits %d\n", (int)fits);
+    if (!fits) return 0;
+  } else {
+    g = go2::plan_grid(s, (size_t)std::atoll(argv[1]), c, std::atoi(argv[3]), std::atoi(argv[4]));
+  }
+  std::printf("scratch_bytes %zu\nframe_bytes %zu\nhelp_bytes %zu\n", go2::scratch_bytes(s, g),
+              go2::frame_bytes(g), go2::help_bytes(s, g));
   std::printf("think %d\ndepth %d\nmaxe %d\nlevels %d\nlds_levels %d\nlds_bytes %zu\n", (int)s.think, s.depth, s.maxe,
               s.levels, s.lds_levels, s.lds_bytes);
   std::printf("grid %zu\nwaves %zu\ntotal_lanes %zu\ng_levels %zu\nK %d\nsplit_min %d\next_min %d\next_lo %d\nhelp %d\n",

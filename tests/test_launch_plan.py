@@ -86,3 +86,34 @@ def test_plan_invariants_random_sweep(plan_bin):
 def test_think_mode_plan(plan_bin):
     p = plan(plan_bin, 1000, 60, think_depth=6)
     assert (p["think"], p["depth"], p["maxe"], p["split_min"], p["help"]) == (1, 60, 8, 64, 0)
+
+
+NOALLOC = 128 << 20  # GO2_NOALLOC_SCRATCH_BYTES
+
+
+def test_noalloc_scratch_fit(plan_bin):
+    # The no-allocation entry point's scratch holds the full grid up to 18
+    # empties; deeper stacks shrink the grid, and every DEPTH fits somehow.
+    p = plan(plan_bin, 100000, 16, capacity=NOALLOC)
+    assert p["fits"] == 1 and p["grid"] == 768 and p["help"] == 1
+    assert p["scratch_bytes"] <= NOALLOC
+    full = plan(plan_bin, 100000, 16)
+    assert full["scratch_bytes"] == p["scratch_bytes"]
+    for depth in range(1, 61):
+        q = plan(plan_bin, 100000, depth, capacity=NOALLOC)
+        assert q["fits"] == 1 and q["scratch_bytes"] <= NOALLOC, depth
+        assert q["grid"] >= 256, depth  # at least one workgroup per CU
+    q = plan(plan_bin, 100000, 32, capacity=NOALLOC)
+    assert q["grid"] < 768  # shrunk
+    q = plan(plan_bin, 100000, 32, capacity=1 << 20)
+    assert q["fits"] == 0
+
+
+def test_scratch_bytes_layout(plan_bin):
+    p = plan(plan_bin, 100000, 16)
+    cells = p["total_lanes"] * p["g_levels"]
+    r = lambda b: (b + 255) // 256 * 256  # noqa: E731
+    assert p["frame_bytes"] == 3 * r(cells * 8) + 2 * r(cells * 4)
+    mb = r((p["maxe"] + 1 - p["ext_lo"]) * p["total_lanes"] * 4)
+    assert p["help_bytes"] == 4096 * 64 + 7 * mb
+    assert p["scratch_bytes"] == 256 + p["help_bytes"] + p["frame_bytes"]
