@@ -82,7 +82,8 @@ class Config(ctypes.Structure):
         ("export_min_empties", ctypes.c_int32),
         ("export_low_empties", ctypes.c_int32),
         ("think_depth", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 5),
+        ("leaf_batch_min", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
     ]
 
 

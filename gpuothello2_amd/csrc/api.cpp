@@ -53,6 +53,7 @@ int resolve_config(const go2_config* cfg, go2_config* out) {
   if (c.think_depth < 0 || c.think_depth > 30) return fail(GO2_E_ARG, "config.think_depth must be in [0, 30]");
   if (c.tt_log2_buckets < 0 || c.tt_log2_buckets > 32)
     return fail(GO2_E_ARG, "config.tt_log2_buckets must be in [0, 32]");
+  if (c.leaf_batch_min < 0) return fail(GO2_E_ARG, "config.leaf_batch_min must be >= 0");
   if (c.split_cut_after < 0 || c.split_cut_after > 3)
     return fail(GO2_E_ARG, "config.split_cut_after must be in [0, 3]");
   *out = c;

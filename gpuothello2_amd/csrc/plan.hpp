@@ -19,6 +19,7 @@ namespace go2 {
 constexpr int kFrameBase = 4;
 constexpr size_t kLastTableBytes = 2048;   // last-square flip counts (LDS)
 constexpr size_t kBase3TableBytes = 2048;  // base-3 table of the evaluator (LDS)
+constexpr size_t kLeafMapBytesPerWave = 64; // leaf batch: worker slot -> requesting lane (LDS)
 constexpr int kDefaultLdsLevels = 6;
 constexpr size_t kFrameBytesPerLevel = 32;  // fb, mv, od (u64) + wd, sx (u32) per lane
 constexpr unsigned int kHelpSlots = 4096;
@@ -56,7 +57,8 @@ inline StackPlan plan_stack(int max_depth, const go2_config& cfg) {
   s.lds_levels = cfg.lds_levels > 0 ? cfg.lds_levels : kDefaultLdsLevels;
   if (s.lds_levels > s.levels) s.lds_levels = s.levels;
   s.lds_bytes = (size_t)s.lds_levels * (size_t)cfg.block_size * kFrameBytesPerLevel + kLastTableBytes +
-                (cfg.ordering == GO2_ORDER_LEARNED || s.think ? kBase3TableBytes : 0);
+                (cfg.ordering == GO2_ORDER_LEARNED || s.think ? kBase3TableBytes : 0) +
+                (size_t)cfg.block_size / 64 * kLeafMapBytesPerWave;
   return s;
 }
 

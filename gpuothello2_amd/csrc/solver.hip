@@ -177,6 +177,8 @@ __device__ __forceinline__ int move_key(int v, bool exact, int sq) {
 }
 __device__ __forceinline__ int key_value(int k) { return (k >> 7) - 64; }
 constexpr int kMovePass = 64, kMoveNone = 65;
+// A leaf batch runs at the latest when its oldest job has waited this many trips.
+constexpr int kLeafAge = 3;
 
 struct SolveArgs {
   const go2_board* boards;
@@ -199,6 +201,8 @@ struct SolveArgs {
   int key_eval_mul;
   int key_mob_mul;
   int max_pops;       // finished nodes a lane returns from per loop trip
+  int leaf_batch;     // nodes with three empties queue their children and the wavefront evaluates
+                      // them together once this many are queued (0 = each lane on its own)
   int think_depth;    // think mode: moves before the evaluator scores a position
   float ev_scale;     // evaluator units per disc
   // transposition table (null = none): probed when pushing children with
@@ -302,6 +306,23 @@ __device__ __forceinline__ unsigned int slot_full(unsigned int g, int e, int sq)
   return ((((g << 5 | (unsigned int)e) << 6) | (unsigned int)sq) << 2) | kSlotFull;
 }
 
+__device__ __forceinline__ uint64_t shfl64(uint64_t x, int src) {
+  const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)x, src);
+  const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(x >> 32), src);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// A queued child of a node with three empties: play sq on (P, O) and score
+// the resulting two-empties position exactly; returns the value for the
+// node's side (the negated child score).
+template <class T>
+__device__ __forceinline__ int leaf_job(uint64_t P, uint64_t O, int sq, const T& lastc, int& nodes) {
+  const uint64_t f = flip(P, O, sq);
+  const uint64_t bit = 1ULL << sq;
+  ++nodes;
+  return -solve2(O ^ f, P ^ f ^ bit, lastc, nodes);
+}
+
 template <class T>
 __device__ __forceinline__ T ld_agent(T* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -353,6 +374,8 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
   uint8_t* lastc = reinterpret_cast<uint8_t*>(l_sx + LL * bs);  // last-square flip counts
   for (int k = tid; k < 2048; k += bs) lastc[k] = (uint8_t)line_flip_count(k >> 8, k & 255);
   uint16_t* t3 = reinterpret_cast<uint16_t*>(lastc + 2048);  // base-3 table (ORDER 2)
+  // leaf batch: worker slot -> requesting lane, 64 bytes per wavefront
+  uint8_t* lmap = lastc + 2048 + (ORDER == 2 ? kBase3Size * 2 : 0) + (tid & ~63);
   if constexpr (ORDER != 2) __syncthreads();
   if constexpr (ORDER == 2) {
     for (int b = tid; b < kBase3Size; b += bs) {
@@ -414,7 +437,9 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
   };
 
   // Lane state.
-  enum { IDLE = 0, SEARCH = 1, WAIT = 2, ABORT = 3, REPORT = 4 };
+  // LEAFWAIT: at a node with three empties whose children are queued for
+  // the wavefront's next leaf batch.
+  enum { IDLE = 0, SEARCH = 1, WAIT = 2, ABORT = 3, REPORT = 4, LEAFWAIT = 5 };
   int mode = IDLE;
   uint64_t P = 0, O = 0, moves = 0, order = 0, to_score = 0, keys = 0;
   int alpha = 0, beta = 0, best = 0, E = 0, Eroot = 0, nkeys = 0;
@@ -489,6 +514,12 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
     pop_pending = false;
     a0mv = (((alpha >> 1) & 127) << 6) | 63;
     if (E >= a.split_min) set_sx(E, sx_init(alpha));
+    if (!THINK && E == 3 && a.leaf_batch && cm) {
+      // every child is queued at once and scored exactly (an exact value is
+      // a valid fail-soft result whatever the window)
+      mode = LEAFWAIT;
+      scoring = false;
+    }
   };
 
   // This lane starts searching child (cP, cO) of frame oe of its owner, with
@@ -510,8 +541,8 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
       alpha = -cb;
       beta = -ca;
       negate = true;
-      enter_node(cm, ct);
       mode = SEARCH;
+      enter_node(cm, ct);
     } else {
       const uint64_t om = mobility(cO, cP);
       if (om) {
@@ -520,8 +551,8 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
         alpha = ca;
         beta = cb;
         negate = false;
-        enter_node(om, ct);
         mode = SEARCH;
+        enter_node(om, ct);
       } else {
         const int rv = -final_score(cP, cO);
         mkey = move_key(rv, rv > ca, csq);
@@ -533,6 +564,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
 
   const unsigned int tt_gen = a.tt ? *a.tt_gen : 0u;
   unsigned int trip = 0;
+  int leaf_age = 0;  // wavefront-uniform: trips since the last leaf batch
   const unsigned long long deadline = __builtin_amdgcn_s_memrealtime() + a.limit_ticks;
   if (a.slots && lane == 0) atomicAdd(&a.ctr->active_waves, 1u);
   for (;;) {
@@ -607,8 +639,8 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
                 owner = -1;
                 mkey = -1;
                 if constexpr (THINK) ebase = E - a.think_depth - 2;
-                enter_node(m, kPV);
                 mode = SEARCH;
+                enter_node(m, kPV);
               }
             }
           }
@@ -723,7 +755,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
           if (!sx_ext(sx)) ext_mask &= ~(1u << (e - 3));
         }
       }
-      if (owner_ext && (mode == SEARCH || mode == WAIT) &&
+      if (owner_ext && (mode == SEARCH || mode == WAIT || mode == LEAFWAIT) &&
           ld_agent(&a.mb_cut[mb_idx(owner_E, (unsigned int)owner)])) {
         mode = ABORT;
         dead_level = 0;
@@ -776,7 +808,8 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
       dm &= dm - 1;
       const int dl = __builtin_amdgcn_readlane(dead_level, l);
       if (lane == l) dead_new = false;
-      if ((mode == SEARCH || mode == WAIT) && !owner_ext && owner == l && (dl == 0 || dl == owner_E)) {
+      if ((mode == SEARCH || mode == WAIT || mode == LEAFWAIT) && !owner_ext && owner == l &&
+          (dl == 0 || dl == owner_E)) {
         mode = ABORT;
         dead_level = 0;
         dead_new = true;
@@ -1101,6 +1134,71 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
       if (helpers_out == 0) {
         report_aborted = true;
         mode = REPORT;
+      }
+    }
+
+    // ---- 5b. leaf batch: the queued children of nodes with three empties ----
+    // Each waiting lane has one job per legal move (at most three).  Once at
+    // least leaf_batch jobs are queued, or the oldest has waited kLeafAge
+    // trips, the whole wavefront evaluates up to 64 of them at once (whole
+    // nodes only): worker w learns its requesting lane from an LDS map,
+    // fetches the board by lane permute, and the requester reads its
+    // children's values back the same way.
+    if constexpr (!THINK) {
+      const uint64_t lw = __ballot(mode == LEAFWAIT);
+      if (lw) {
+        ++leaf_age;
+        const int cnt = mode == LEAFWAIT ? popc64(moves) : 0;  // 1..3
+        const uint64_t lt = __lanemask_lt();
+        const uint64_t c1 = __ballot(cnt & 1), c2 = __ballot(cnt & 2);
+        const int jobs = popc64(c1) + 2 * popc64(c2);
+        if (jobs >= a.leaf_batch || leaf_age > kLeafAge) {
+          leaf_age = 0;
+          const int off = popc64(c1 & lt) + 2 * popc64(c2 & lt);  // first job slot of this lane
+          const bool sel = cnt > 0 && off + cnt <= 64;
+          const uint64_t selm = __ballot(sel);
+          const int nsel = (int)(64 - __builtin_clzll(selm));     // requesters are a lane prefix
+          const int njobs = __builtin_amdgcn_readlane(off + cnt, nsel - 1);
+          if (sel)
+            for (int k = 0; k < cnt; ++k) lmap[off + k] = (uint8_t)lane;
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          const int r = lane < njobs ? (int)lmap[lane] : lane;
+          const uint64_t jP = shfl64(P, r), jO = shfl64(O, r), jm = shfl64(moves, r);
+          const int k = lane - __shfl(off, r);
+          int jv = 0;
+          if (lane < njobs) {
+            uint64_t m = jm;
+            for (int i = 0; i < k; ++i) m &= m - 1;
+            int nl = 0;
+            jv = leaf_job(jP, jO, ctz64(m), lastc, nl);
+            c_nodes += (unsigned int)nl;
+          }
+          // results back to the requesters
+          int vk[3];
+#pragma unroll
+          for (int i = 0; i < 3; ++i) vk[i] = __shfl(jv, min(off + i, 63));
+          if (sel) {
+            uint64_t m = moves;
+            int nb = -64;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+              if (i < cnt) {
+                const int sq = ctz64(m);
+                m &= m - 1;
+                if (E == Eroot && owner < 0) mkey = max(mkey, move_key(vk[i], true, sq));
+                nb = max(nb, vk[i]);
+              }
+            }
+            best = max(best, nb);
+            alpha = max(alpha, best);
+            ndone = min(ndone + cnt, 3);
+            moves = 0;
+            mode = SEARCH;
+            commit = true;
+          }
+        }
       }
     }
 
@@ -1471,6 +1569,7 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   args.total_lanes = (unsigned int)total_lanes;
   args.limit_ticks = 0;
   args.max_pops = cfg.max_pops_per_trip > 0 ? cfg.max_pops_per_trip : 2;
+  args.leaf_batch = sp.think || cfg.leaf_batch_min > 64 ? 0 : (cfg.leaf_batch_min > 0 ? cfg.leaf_batch_min : 48);
   if (gp.help) {
     const unsigned int nslots = kHelpSlots;
     const size_t mb = mailbox_bytes(sp, gp);

@@ -90,7 +90,11 @@ typedef struct go2_config {
   int32_t think_depth;          /* > 0: the reference's think mode instead of the exact solve: the
                                    position after this many moves is scored by the installed
                                    evaluator (go2_think) */
-  int32_t reserved[5];
+  int32_t leaf_batch_min;       /* nodes with three empties queue all their children, and a
+                                   wavefront scores queued children together once this many
+                                   (1..64) are waiting; 0 = auto (48), > 64 = each lane scores its
+                                   own, one child per loop trip */
+  int32_t reserved[4];
 } go2_config;
 
 /* Counters of one solve.  Replaces the reference's `total` node counter

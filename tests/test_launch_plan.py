@@ -32,12 +32,13 @@ def plan(binary, n, depth, cus=256, per_cu=3, **fields):
 def test_headline_shape(plan_bin):
     p = plan(plan_bin, 100000, 16)
     assert (p["depth"], p["maxe"], p["levels"], p["lds_levels"]) == (16, 16, 13, 6)
-    assert p["lds_bytes"] == 6 * 256 * 32 + 2048
+    assert p["lds_bytes"] == 6 * 256 * 32 + 2048 + 4 * 64
     assert (p["grid"], p["waves"], p["total_lanes"]) == (768, 3072, 768 * 256)
     assert p["K"] == -(-100000 // 3072)
     assert (p["help"], p["ext_min"], p["ext_lo"], p["split_min"]) == (1, 12, 9, 8)
     p = plan(plan_bin, 100000, 16, ordering=2, positions_per_wave=16)
-    assert p["lds_bytes"] == 6 * 256 * 32 + 2048 + 2048 and p["K"] == 16
+    assert p["lds_bytes"] == 6 * 256 * 32 + 2048 + 2048 + 4 * 64 and p["K"] == 16
+    assert 3 * p["lds_bytes"] <= 160 * 1024  # three workgroups per CU
 
 
 def test_depth_beyond_kernel_limit_sizes_for_32(plan_bin):
