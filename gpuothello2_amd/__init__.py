@@ -83,7 +83,8 @@ class Config(ctypes.Structure):
         ("export_low_empties", ctypes.c_int32),
         ("think_depth", ctypes.c_int32),
         ("leaf_batch_min", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 4),
+        ("score_batch_min", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 3),
     ]
 
 

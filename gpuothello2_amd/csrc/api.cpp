@@ -54,6 +54,7 @@ int resolve_config(const go2_config* cfg, go2_config* out) {
   if (c.tt_log2_buckets < 0 || c.tt_log2_buckets > 32)
     return fail(GO2_E_ARG, "config.tt_log2_buckets must be in [0, 32]");
   if (c.leaf_batch_min < 0) return fail(GO2_E_ARG, "config.leaf_batch_min must be >= 0");
+  if (c.score_batch_min < 0) return fail(GO2_E_ARG, "config.score_batch_min must be >= 0");
   if (c.split_cut_after < 0 || c.split_cut_after > 3)
     return fail(GO2_E_ARG, "config.split_cut_after must be in [0, 3]");
   *out = c;

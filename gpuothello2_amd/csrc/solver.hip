@@ -179,6 +179,8 @@ __device__ __forceinline__ int key_value(int k) { return (k >> 7) - 64; }
 constexpr int kMovePass = 64, kMoveNone = 65;
 // A leaf batch runs at the latest when its oldest job has waited this many trips.
 constexpr int kLeafAge = 3;
+// ... and a score batch when its oldest node has waited this many.
+constexpr int kScoreAge = 3;
 
 struct SolveArgs {
   const go2_board* boards;
@@ -203,6 +205,8 @@ struct SolveArgs {
   int max_pops;       // finished nodes a lane returns from per loop trip
   int leaf_batch;     // nodes with three empties queue their children and the wavefront evaluates
                       // them together once this many are queued (0 = each lane on its own)
+  int score_batch;    // nodes that order their children queue them and the wavefront computes
+                      // the keys together once this many are queued (0 = one child per trip)
   int think_depth;    // think mode: moves before the evaluator scores a position
   float ev_scale;     // evaluator units per disc
   // transposition table (null = none): probed when pushing children with
@@ -439,7 +443,9 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
   // Lane state.
   // LEAFWAIT: at a node with three empties whose children are queued for
   // the wavefront's next leaf batch.
-  enum { IDLE = 0, SEARCH = 1, WAIT = 2, ABORT = 3, REPORT = 4, LEAFWAIT = 5 };
+  // SCOREWAIT: at a node whose children are queued for the wavefront's next
+  // score batch (their ordering keys).
+  enum { IDLE = 0, SEARCH = 1, WAIT = 2, ABORT = 3, REPORT = 4, LEAFWAIT = 5, SCOREWAIT = 6 };
   int mode = IDLE;
   uint64_t P = 0, O = 0, moves = 0, order = 0, to_score = 0, keys = 0;
   int alpha = 0, beta = 0, best = 0, E = 0, Eroot = 0, nkeys = 0;
@@ -499,6 +505,22 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
     return ((size_t)(e - a.ext_lo) * kExtRec + (size_t)k) * a.total_lanes + g;
   };
 
+  // Ordering key of the child (cP, cO) with replies cm, reached by the move
+  // `bit` from a node with E empties: fastest first (fewest replies),
+  // corners preferred, X-squares delayed; near the root the learned
+  // evaluator of the child joins the key.
+  auto move_key_of = [&](uint64_t cP, uint64_t cO, uint64_t cm, uint64_t bit, int e) -> int {
+    int k = min(popc64(cm), 15) * 4 + ((bit & kCorners) ? 0 : 2) + ((bit & kXSquares) ? 1 : 0);
+    if constexpr (ORDER == 2) {
+      if (e >= a.learned_min) {
+        int ev = a.ev_bias[e - 1];
+        for_each_feature(cP, cO, t3, [&](int, uint32_t fi) { ev += a.ev_w[fi]; });
+        k = min(max(((ev * a.key_eval_mul) >> 12) + popc64(cm) * a.key_mob_mul + 32, 0), 63);
+      }
+    }
+    return k;
+  };
+
   // A node at level E with legal moves `cm` becomes the current node.
   auto enter_node = [&](uint64_t cm, int type) {
     moves = cm;
@@ -514,6 +536,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
     pop_pending = false;
     a0mv = (((alpha >> 1) & 127) << 6) | 63;
     if (E >= a.split_min) set_sx(E, sx_init(alpha));
+    if (!THINK && scoring && a.score_batch) mode = SCOREWAIT;
     if (!THINK && E == 3 && a.leaf_batch && cm) {
       // every child is queued at once and scored exactly (an exact value is
       // a valid fail-soft result whatever the window)
@@ -564,7 +587,8 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
 
   const unsigned int tt_gen = a.tt ? *a.tt_gen : 0u;
   unsigned int trip = 0;
-  int leaf_age = 0;  // wavefront-uniform: trips since the last leaf batch
+  int leaf_age = 0;   // wavefront-uniform: trips since the last leaf batch
+  int score_age = 0;  // wavefront-uniform: trips since the last score batch
   const unsigned long long deadline = __builtin_amdgcn_s_memrealtime() + a.limit_ticks;
   if (a.slots && lane == 0) atomicAdd(&a.ctr->active_waves, 1u);
   for (;;) {
@@ -755,7 +779,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
           if (!sx_ext(sx)) ext_mask &= ~(1u << (e - 3));
         }
       }
-      if (owner_ext && (mode == SEARCH || mode == WAIT || mode == LEAFWAIT) &&
+      if (owner_ext && (mode == SEARCH || mode == WAIT || mode == LEAFWAIT || mode == SCOREWAIT) &&
           ld_agent(&a.mb_cut[mb_idx(owner_E, (unsigned int)owner)])) {
         mode = ABORT;
         dead_level = 0;
@@ -808,7 +832,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
       dm &= dm - 1;
       const int dl = __builtin_amdgcn_readlane(dead_level, l);
       if (lane == l) dead_new = false;
-      if ((mode == SEARCH || mode == WAIT || mode == LEAFWAIT) && !owner_ext && owner == l &&
+      if ((mode == SEARCH || mode == WAIT || mode == LEAFWAIT || mode == SCOREWAIT) && !owner_ext && owner == l &&
           (dl == 0 || dl == owner_E)) {
         mode = ABORT;
         dead_level = 0;
@@ -1021,17 +1045,8 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
           passed = true;
         }
       } else if (scoring) {
-        // Fastest first: fewest replies, corners preferred, X-squares delayed.
-        // Near the root the learned evaluator of the child joins the key.
         ++c_steps;
-        int k = min(popc64(cm), 15) * 4 + ((bit & kCorners) ? 0 : 2) + ((bit & kXSquares) ? 1 : 0);
-        if constexpr (ORDER == 2) {
-          if (E >= a.learned_min) {
-            int ev = a.ev_bias[E - 1];
-            for_each_feature(cP, cO, t3, [&](int, uint32_t fi) { ev += a.ev_w[fi]; });
-            k = min(max(((ev * a.key_eval_mul) >> 12) + popc64(cm) * a.key_mob_mul + 32, 0), 63);
-          }
-        }
+        const int k = move_key_of(cP, cO, cm, bit, E);
         const uint64_t valid = (1ULL << (7 * nkeys)) - 1;
         const uint64_t le = ((((uint64_t)k * kLo7) | kHi7) - keys) & kHi7 & valid;
         const int c = popc64(le);  // entries with key <= k: insert after them (stable)
@@ -1197,6 +1212,72 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
             moves = 0;
             mode = SEARCH;
             commit = true;
+          }
+        }
+      }
+    }
+
+    // ---- 5c. score batch: the children of nodes that order their moves ----
+    // Each waiting lane has one job per legal move (its to_score set).  A
+    // worker plays its move, counts the replies and computes the key; the
+    // node's jobs are ranked by (key, square) through lane permutes, and the
+    // node's packed order list takes the first nine: the list the one-child-
+    // per-trip scoring builds by stable insertion.
+    if constexpr (!THINK) {
+      const uint64_t swm = __ballot(mode == SCOREWAIT);
+      if (swm) {
+        ++score_age;
+        const int cnt = mode == SCOREWAIT ? popc64(to_score) : 0;  // 2..32
+        const uint64_t lt = __lanemask_lt();
+        int off = 0, jobs = 0;
+#pragma unroll
+        for (int b = 0; b < 6; ++b) {
+          const uint64_t bb = __ballot((cnt >> b) & 1);
+          off += popc64(bb & lt) << b;
+          jobs += popc64(bb) << b;
+        }
+        if (jobs >= a.score_batch || score_age > kScoreAge) {
+          score_age = 0;
+          const bool sel = cnt > 0 && off + cnt <= 64;
+          const uint64_t selm = __ballot(sel);
+          const int njobs = __builtin_amdgcn_readlane(off + cnt, 63 - __builtin_clzll(selm));
+          if (sel)
+            for (int k = 0; k < cnt; ++k) lmap[off + k] = (uint8_t)lane;
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          const bool worker = lane < njobs;
+          const int r = worker ? (int)lmap[lane] : lane;
+          const uint64_t jP = shfl64(P, r), jO = shfl64(O, r), jm = shfl64(to_score, r);
+          const int jE = __shfl(E, r), joff = __shfl(off, r), jcnt = __shfl(cnt, r);
+          const int k = lane - joff;
+          int key = 0, jsq = 0;
+          if (worker) {
+            uint64_t m = jm;
+            for (int i = 0; i < k; ++i) m &= m - 1;
+            jsq = ctz64(m);
+            const uint64_t bit = 1ULL << jsq;
+            const uint64_t f = flip(jP, jO, jsq);
+            const uint64_t cP = jO ^ f, cO = jP ^ f ^ bit;
+            key = move_key_of(cP, cO, mobility(cP, cO), bit, jE);
+            ++c_steps;
+          }
+          int rank = 0;
+          for (int j = 0; __ballot(worker && j < jcnt); ++j) {
+            const int kj = __shfl(key, min(joff + j, 63));
+            if (worker && j < jcnt && (kj < key || (kj == key && j < k))) ++rank;
+          }
+          const uint64_t contrib = worker && rank < 9 ? (uint64_t)(jsq + 1) << (7 * rank) : 0;
+          uint64_t ord = 0;
+          for (int j = 0; __ballot(sel && j < cnt); ++j) {
+            const uint64_t c = shfl64(contrib, min(off + j, 63));
+            if (sel && j < cnt) ord |= c;
+          }
+          if (sel) {
+            order = ord;
+            to_score = 0;
+            scoring = false;
+            mode = SEARCH;
           }
         }
       }
@@ -1570,6 +1651,7 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   args.limit_ticks = 0;
   args.max_pops = cfg.max_pops_per_trip > 0 ? cfg.max_pops_per_trip : 2;
   args.leaf_batch = sp.think || cfg.leaf_batch_min > 64 ? 0 : (cfg.leaf_batch_min > 0 ? cfg.leaf_batch_min : 48);
+  args.score_batch = sp.think || cfg.score_batch_min > 64 ? 0 : (cfg.score_batch_min > 0 ? cfg.score_batch_min : 32);
   if (gp.help) {
     const unsigned int nslots = kHelpSlots;
     const size_t mb = mailbox_bytes(sp, gp);

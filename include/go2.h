@@ -94,7 +94,11 @@ typedef struct go2_config {
                                    wavefront scores queued children together once this many
                                    (1..64) are waiting; 0 = auto (48), > 64 = each lane scores its
                                    own, one child per loop trip */
-  int32_t reserved[4];
+  int32_t score_batch_min;      /* nodes that order their moves queue every child, and a
+                                   wavefront computes the queued ordering keys together once
+                                   this many (1..64) are waiting; 0 = auto (32), > 64 = each lane
+                                   scores its own, one child per loop trip */
+  int32_t reserved[3];
 } go2_config;
 
 /* Counters of one solve.  Replaces the reference's `total` node counter
