@@ -148,7 +148,7 @@ void go2_config_default(go2_config* cfg) {
   std::memset(cfg, 0, sizeof(*cfg));
   cfg->ordering = GO2_ORDER_MOBILITY;
   cfg->mobility_min_empties = 7;
-  cfg->learned_min_empties = 12;
+  cfg->learned_min_empties = 11;
   cfg->block_size = 256;
   cfg->blocks_per_cu = 0;
   cfg->positions_per_wave = 0;

@@ -96,7 +96,7 @@ typedef struct go2_config {
                                    own, one child per loop trip */
   int32_t score_batch_min;      /* nodes that order their moves queue every child, and a
                                    wavefront computes the queued ordering keys together once
-                                   this many (1..64) are waiting; 0 = auto (32), > 64 = each lane
+                                   this many (1..64) are waiting; 0 = auto (64), > 64 = each lane
                                    scores its own, one child per loop trip */
   int32_t reserved[3];
 } go2_config;

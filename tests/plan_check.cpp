@@ -15,7 +15,7 @@ static void defaults(go2_config* c) {
   std::memset(c, 0, sizeof(*c));
   c->ordering = GO2_ORDER_MOBILITY;
   c->mobility_min_empties = 7;
-  c->learned_min_empties = 12;
+  c->learned_min_empties = 11;
   c->block_size = 256;
   c->time_limit_ms = 600000;
 }
