@@ -155,7 +155,7 @@ def main() -> int:
         task.wait()
         log(f"[bench] warmup {i + 1}/{args.warmup} done ({wn} positions)")
     task.upload(boards)
-    kernel_ms, nodes, step_scores = [], [], []
+    kernel_ms, nodes, step_scores, step_stats = [], [], [], []
     t0 = time.perf_counter()
     for i in range(args.steps):
         task.launch()
@@ -164,6 +164,7 @@ def main() -> int:
         kernel_ms.append(st["kernel_ms"])
         nodes.append(st["nodes"])
         step_scores.append(scores)
+        step_stats.append(st)
         log(f"[bench] step {i + 1}/{args.steps}: kernel {st['kernel_ms']:.1f} ms, nodes {st['nodes']}")
     t1 = time.perf_counter()
     # download() (n bytes of scores + counters) sits inside the timed loop.
@@ -200,9 +201,9 @@ def main() -> int:
         for name, (c, knobs) in legs.items():
             c.time_limit_ms = limit
             try:
-                s, st = run_leg(b, E, c, f"A/B {name}")
-                ab[name] = leg_summary(an, st, knobs)
-                got[name] = s
+                leg_scores, leg_st = run_leg(b, E, c, f"A/B {name}")
+                ab[name] = leg_summary(an, leg_st, knobs)
+                got[name] = leg_scores
             except go2.Go2Error as e:
                 log(f"[bench] A/B {name} did not finish: {e}")
                 ab[name] = {"value_upper_bound": an / args.ab_limit_s, "knobs": knobs, "error": str(e)}
@@ -276,7 +277,7 @@ def main() -> int:
             "workload": f"exact solve of {n} random-playout positions at {E} empties",
             "positions": n, "empties": E, **head_knobs,
             "nodes_per_position": nodes_per_launch / n,
-            "table": {k_: st[k_] for k_ in ("tt_probes", "tt_hits", "tt_cuts", "tt_stores")},
+            "table": {k_: step_stats[-1][k_] for k_ in ("tt_probes", "tt_hits", "tt_cuts", "tt_stores")},
             "ab": ab,
             "score_match": match,
             "evaluator": {k_: learned_meta[k_] for k_ in ("positions", "empties", "split", "valid_mse",
