@@ -47,6 +47,7 @@ EXPORTED_SYMBOLS = (
     "go2_generate_device", "go2_pattern_info", "go2_pattern_features_device",
     "go2_pattern_eval_device", "go2_sgd_gradient_device", "go2_sgd_apply_device",
     "go2_quantize_weights_device", "go2_set_evaluator", "go2_solve_device_noalloc",
+    "go2_task_upload_async", "go2_host_alloc", "go2_host_free",
 )
 NOALLOC_SCRATCH_BYTES = 128 << 20
 
@@ -144,6 +145,10 @@ def lib() -> ctypes.CDLL:
     L.go2_task_create.restype = vp
     L.go2_task_upload.argtypes = [vp, vp, sz]
     L.go2_task_launch.argtypes = [vp]
+    L.go2_task_upload_async.argtypes = [vp, vp, sz]
+    L.go2_host_alloc.argtypes = [sz, ctypes.POINTER(ctypes.c_void_p)]
+    L.go2_host_free.argtypes = [vp]
+    L.go2_host_free.restype = None
     L.go2_task_query.argtypes = [vp]
     L.go2_task_wait.argtypes = [vp]
     L.go2_task_download.argtypes = [vp, vp, ctypes.POINTER(Stats)]
@@ -162,7 +167,7 @@ def lib() -> ctypes.CDLL:
     for name in EXPORTED_SYMBOLS:
         fn = getattr(L, name)
         if name not in ("go2_last_error", "go2_config_default", "go2_board_to_text",
-                        "go2_task_create", "go2_task_destroy"):
+                        "go2_task_create", "go2_task_destroy", "go2_host_free"):
             fn.restype = i32
     _lib = L
     return L

@@ -8,7 +8,9 @@
 //     (main.cu:49-63); a missing line prints "Failed to load boards" and exits 1;
 //   * boards sorted and de-duplicated (main.cu:64-65);
 //   * batches of at most 2,000,000 boards, each on its own stream, all launched
-//     before polling (main.cu:130-155);
+//     before polling (main.cu:130-155); each batch's upload is queued on its
+//     stream from page-locked memory ahead of its solve, so batch k+1 copies
+//     while batch k solves (GO2_CLI_BATCH overrides the batch size);
 //   * stderr lines "n = ...", "start!", the status/elapsed/table line and
 //     "total nodes: ..." (main.cu:129,144,158-162,172);
 //   * usage text and exit code 1 for fewer than three arguments (main.cu:176-179).
@@ -53,7 +55,7 @@ struct FileCloser {
 };
 using File = std::unique_ptr<FILE, FileCloser>;
 
-constexpr size_t kBatch = 2000000;
+constexpr size_t kDefaultBatch = 2000000;  // main.cu:130
 
 using go2cli::die;
 
@@ -192,19 +194,33 @@ int main(int argc, char** argv) {
   // reference's think-mode moves (main.cu:31-39,114-117).
   const bool with_moves = think || env_int("GO2_BEST_MOVE", 0) != 0;
   const size_t n = boards.size();
+  const long batch_env = env_int("GO2_CLI_BATCH", 0);
+  const size_t kBatch = batch_env > 0 ? (size_t)batch_env : kDefaultBatch;
   fprintf(stderr, "n = %zu\n", n);
+  // Boards in page-locked memory, so that every batch's copy is queued
+  // without waiting and overlaps the solves of the batches before it.
+  go2_board* pinned = nullptr;
+  if (n) {
+    void* p = nullptr;
+    if (go2_host_alloc(n * sizeof(go2_board), &p) != GO2_OK) die(2, go2_last_error());
+    pinned = static_cast<go2_board*>(p);
+    go2cli::parallel_ranges(n, go2cli::worker_count(n), [&](unsigned, size_t lo, size_t hi) {
+      std::memcpy(pinned + lo, boards.data() + lo, (hi - lo) * sizeof(go2_board));
+    });
+  }
   std::vector<go2_task*> tasks;
   for (size_t i = 0; i < n; i += kBatch) {
-    const size_t cnt = std::min(kBatch, n - i);
-    go2_task* t = go2_task_create(cnt, think ? 60 : (int)depth, &cfg);
+    go2_task* t = go2_task_create(std::min(kBatch, n - i), think ? 60 : (int)depth, &cfg);
     if (!t) die(2, go2_last_error());
-    if (go2_task_upload(t, boards.data() + i, cnt) != GO2_OK) die(2, go2_last_error());
     tasks.push_back(t);
   }
   fprintf(stderr, "start!\n");
   const auto t0 = std::chrono::steady_clock::now();
-  for (go2_task* t : tasks)
-    if (go2_task_launch(t) != GO2_OK) die(2, go2_last_error());
+  for (size_t k = 0; k < tasks.size(); ++k) {
+    const size_t i = k * kBatch, cnt = std::min(kBatch, n - i);
+    if (go2_task_upload_async(tasks[k], pinned + i, cnt) != GO2_OK) die(2, go2_last_error());
+    if (go2_task_launch(tasks[k]) != GO2_OK) die(2, go2_last_error());
+  }
   for (;;) {
     bool done = true;
     for (go2_task* t : tasks) {
@@ -236,6 +252,7 @@ int main(int argc, char** argv) {
           "hipSuccess (no error), elapsed: %ldms, table update count: %" PRIu64 ", table hit: %" PRIu64
           ", table find: %" PRIu64 "\n",
           ms, tt_stores, tt_hits, tt_probes);
+  go2_host_free(pinned);
   go2cli::write_results(out.get(), boards, scores.data(), with_moves ? moves.data() : nullptr,
                         think ? think_move_text : move_text);
   fprintf(stderr, "total nodes: %" PRIu64 "\n", nodes);

@@ -293,6 +293,36 @@ int go2_task_upload(go2_task* t, const go2_board* boards, size_t count) {
   return GO2_OK;
 }
 
+int go2_task_upload_async(go2_task* t, const go2_board* boards, size_t count) {
+  if (!t) return fail(GO2_E_ARG, "null task");
+  if (count > t->n) return fail(GO2_E_ARG, "upload count exceeds the task size");
+  if (count && !boards) return fail(GO2_E_ARG, "null boards");
+  if (count) {
+    const hipError_t e = hipMemcpyAsync(t->bufs.boards, boards, count * sizeof(go2_board),
+                                        hipMemcpyHostToDevice, t->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(boards)");
+  }
+  t->count = count;
+  t->upload_max_e = max_empties(boards, count);
+  t->launched = false;
+  return GO2_OK;
+}
+
+int go2_host_alloc(size_t bytes, void** out) {
+  if (!out) return fail(GO2_E_ARG, "null output");
+  *out = nullptr;
+  const hipError_t e = hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    *out = nullptr;
+    return fail(GO2_E_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+  }
+  return GO2_OK;
+}
+
+void go2_host_free(void* ptr) {
+  if (ptr) (void)hipHostFree(ptr);
+}
+
 int go2_task_launch(go2_task* t) {
   if (!t) return fail(GO2_E_ARG, "null task");
   int rc = GO2_OK;

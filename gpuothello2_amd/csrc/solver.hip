@@ -403,6 +403,10 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
   auto lds_k = [&](int E) { return (E - ebase - kFrameBase) * bs + tid; };
   auto hbm_k = [&](int E) { return (size_t)(E - ebase - kFrameBase - LL) * a.total_lanes + gtid; };
   auto in_lds = [&](int E) { return E - ebase - kFrameBase < LL; };
+  // Split words: only the owning lane reads or writes them, and every frame
+  // a sibling is taken from is an ancestor of the owner's current node, so
+  // the current node's word lives in a register (cur_sx, below) and memory
+  // holds the ancestors' words.
   auto get_sx = [&](int E) -> uint32_t { return in_lds(E) ? l_sx[lds_k(E)] : hbm_ld(&a.g_sx[hbm_k(E)]); };
   auto set_sx = [&](int E, uint32_t v) {
     if (in_lds(E)) {
@@ -449,6 +453,16 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
   int mode = IDLE;
   uint64_t P = 0, O = 0, moves = 0, order = 0, to_score = 0, keys = 0;
   int alpha = 0, beta = 0, best = 0, E = 0, Eroot = 0, nkeys = 0;
+  uint32_t cur_sx = 0;  // split word of the current node (levels >= split_min)
+  // split word of level e: the register for the current level, else memory
+  auto sx_of = [&](int e) -> uint32_t { return e == E ? cur_sx : get_sx(e); };
+  auto sx_put = [&](int e, uint32_t v) {
+    if (e == E) {
+      cur_sx = v;
+    } else {
+      set_sx(e, v);
+    }
+  };
   bool scoring = false, negate = false;
   bool pchk = false;            // a child without moves: decide pass / game over next trip
   int pchk_sq = 0;
@@ -535,7 +549,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
     pchk = false;
     pop_pending = false;
     a0mv = (((alpha >> 1) & 127) << 6) | 63;
-    if (E >= a.split_min) set_sx(E, sx_init(alpha));
+    if (E >= a.split_min) cur_sx = sx_init(alpha);
     if (!THINK && scoring && a.score_batch) mode = SCOREWAIT;
     if (!THINK && E == 3 && a.leaf_batch && cm) {
       // every child is queued at once and scored exactly (an exact value is
@@ -735,7 +749,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
           const int e = __builtin_ctz(em) + 3;
           em &= em - 1;
           const size_t m = mb_idx(e, gtid);
-          uint32_t sx = get_sx(e);
+          uint32_t sx = sx_of(e);
           const uint32_t sx0 = sx;
           const unsigned int got =
               __hip_atomic_exchange(&a.mb_done[m], 0u, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
@@ -775,7 +789,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
             }
             if (waiting && alpha < beta && moves) mode = SEARCH;
           }
-          if (sx != sx0) set_sx(e, sx);
+          if (sx != sx0) sx_put(e, sx);
           if (!sx_ext(sx)) ext_mask &= ~(1u << (e - 3));
         }
       }
@@ -806,12 +820,12 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
       const int key = __builtin_amdgcn_readlane(mkey, r);
       const int ab = __builtin_amdgcn_readlane((int)report_aborted, r);
       if (lane == o) {
-        uint32_t sx = get_sx(oe) - 1u;
+        uint32_t sx = sx_of(oe) - 1u;
         if (!ab) {
           sx = sx_with_best(sx, key_value(key));
           if (oe == Eroot && owner < 0) mkey = max(mkey, key);
         }
-        set_sx(oe, sx);
+        sx_put(oe, sx);
         --helpers_out;
       }
       if (lane == r) mode = IDLE;
@@ -1110,7 +1124,8 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
         const int thr = ntype == kCut ? (tail ? 1 : a.cut_after) : (tail && ntype == kAll ? 0 : 1);
         if (E >= a.split_min && moves && ndone >= thr) split_mask |= 1u << (E - 3);
         const int ct = child_type(ntype, ndone);
-        if (a.tt && E >= a.tt_min) set_sx(E, (get_sx(E) & ~(63u << 23)) | ((uint32_t)bestmv_of() << 23));
+        if (E >= a.split_min)
+          set_sx(E, a.tt && E >= a.tt_min ? (cur_sx & ~(63u << 23)) | ((uint32_t)bestmv_of() << 23) : cur_sx);
         if (passed) {
           P = cO;
           O = cP;
@@ -1136,7 +1151,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
       commit = true;
     } else if (mode == WAIT) {
       // Waiting for helpers of the current node; stop them once it is cut.
-      const uint32_t sx = get_sx(E);
+      const uint32_t sx = cur_sx;
       if (sx_pending(sx) == 0) {
         mode = SEARCH;
         commit = true;
@@ -1287,7 +1302,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
     if (commit) {
       for (int pops = 0;; ++pops) {
         if (E >= a.split_min) {
-          const uint32_t sx = get_sx(E);
+          const uint32_t sx = cur_sx;
           const int pd = sx_pending(sx);
           best = max(best, sx_best(sx));
           alpha = max(alpha, best);
@@ -1359,6 +1374,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
         alpha = max(palpha, best);
         if (E >= a.split_min) {
           const uint32_t psx = get_sx(E);
+          cur_sx = psx;
           a0mv = (((sx_alpha0(psx) >> 1) & 127) << 6) | (vv > pbest ? (int)((w >> 21) & 63) : (int)((psx >> 23) & 63));
         }
         if constexpr (THINK) {

@@ -204,6 +204,15 @@ go2_task *go2_task_create(size_t n, int max_depth, const go2_config *cfg);
 /* Copy boards [0, count) into the task (the reference's `abp[j] = ...`,
  * main.cu:138-141).  count <= n. */
 int go2_task_upload(go2_task *task, const go2_board *boards, size_t count);
+/* As go2_task_upload, but only enqueued on the task's stream: the copy runs
+ * while earlier work on other streams (another task's solve) still runs,
+ * and a go2_task_launch after it starts once the copy is done.  `boards`
+ * must stay valid and unchanged until the task has finished (go2_task_wait
+ * or go2_task_query == 1); only memory from go2_host_alloc (pinned) is
+ * copied without the caller waiting.  Replaces the host writes into the
+ * task's managed input array (reference main.cu:138-141) for pipelines
+ * that upload batch k+1 while batch k solves (main.cu:135-155). */
+int go2_task_upload_async(go2_task *task, const go2_board *boards, size_t count);
 /* Enqueue the solve asynchronously (reference solver.cu:330-333). */
 int go2_task_launch(go2_task *task);
 /* 1 when finished, 0 while running, negative on error (reference
@@ -219,6 +228,12 @@ int go2_task_download(go2_task *task, int8_t *scores, go2_stats *stats);
 int go2_task_download_moves(go2_task *task, uint8_t *moves);
 /* Free device buffers and the stream (reference solver.cu:339-348). */
 void go2_task_destroy(go2_task *task);
+
+/* Page-locked host memory for go2_task_upload_async (replaces the managed
+ * input array the reference allocates per batch, solver.cu:317-319);
+ * go2_host_free releases it.  GO2_E_NOMEM when the allocation fails. */
+int go2_host_alloc(size_t bytes, void **out);
+void go2_host_free(void *ptr);
 
 /* ---- synthetic positions -------------------------------------------------------- */
 /* Random legal playouts from the standard start (passing when forced) to a

@@ -150,7 +150,59 @@ def c5(scale: float) -> dict:
             "no_table_prefix_match": bool(np.array_equal(s[:k], s2))}
 
 
-RUNNERS = {"c1": c1, "c2": c2, "c3": c3, "c4": c4, "c5": c5}
+def readme(scale: float) -> dict:
+    """The reference README's run (README.md:22-24): 1,091,780 unique boards at
+    10 empties through the command line, DEPTH 10.  Its boards are not
+    published, so random-playout boards of the same count and depth stand in;
+    the reference's GPU is not named."""
+    import subprocess
+    import tempfile
+
+    from gpuothello2_amd import SOLVER_BIN
+
+    n = max(1000, int(1091780 * scale))
+    b = generate(1091780, 6, int(n * 1.02) + 64, 10)
+    pairs = np.unique(_pairs(b), axis=0)[:n]
+    assert len(pairs) == n, "not enough unique boards"
+    rng = np.random.default_rng(1)
+    pairs = pairs[rng.permutation(n)]  # input order is not sorted; the CLI sorts
+    out = {"config": "README scale", "positions": n, "empties": 10, "reference_elapsed_s": 4.623645,
+           "reference_positions_per_s": 1091780 / 4.623645}
+    with tempfile.TemporaryDirectory() as d:
+        inp, res = os.path.join(d, "in.txt"), os.path.join(d, "out.txt")
+        chars = _base81(pairs)
+        with open(inp, "wb") as f:
+            f.write(f"{n}\n".encode())
+            f.write(chars)
+        for label, env in (("default", {}), ("static", {"GO2_ORDERING": "static"})):
+            t0 = time.perf_counter()
+            r = subprocess.run([SOLVER_BIN, inp, res, "10"], capture_output=True, text=True,
+                               env={"PATH": "/usr/bin:/bin", **env})
+            wall = time.perf_counter() - t0
+            assert r.returncode == 0, r.stderr
+            line = next(ln for ln in r.stderr.splitlines() if "elapsed:" in ln)
+            ms = int(line.split("elapsed: ")[1].split("ms")[0])
+            out[label] = {"elapsed_ms": ms, "positions_per_s": n / (ms / 1e3), "process_wall_s": wall,
+                          "status": line}
+            log(f"[readme] {label}: {line}")
+    return out
+
+
+def _base81(pairs: np.ndarray) -> bytes:
+    """Base81 lines (reference test.cpp:7-19) of (n, 2) boards."""
+    p, o = pairs[:, 0], pairs[:, 1]
+    out = np.empty((len(pairs), 17), dtype=np.uint8)
+    for i in range(16):
+        v = np.full(len(pairs), 33, dtype=np.int64)
+        for j, w in enumerate((1, 3, 9, 32)):
+            sq = np.uint64(4 * i + j)
+            v += ((((p >> sq) & np.uint64(1)) + 2 * ((o >> sq) & np.uint64(1))).astype(np.int64)) * w
+        out[:, i] = v.astype(np.uint8)
+    out[:, 16] = ord("\n")
+    return out.tobytes()
+
+
+RUNNERS = {"c1": c1, "c2": c2, "c3": c3, "c4": c4, "c5": c5, "readme": readme}
 
 if __name__ == "__main__":
     args = [a for a in sys.argv[1:] if not a.startswith("--")]
@@ -158,6 +210,6 @@ if __name__ == "__main__":
     if "--scale" in sys.argv:
         scale = float(sys.argv[sys.argv.index("--scale") + 1])
         args = [a for a in args if a != sys.argv[sys.argv.index("--scale") + 1]]
-    for name in args or list(RUNNERS):
+    for name in args or ["c1", "c2", "c3", "c4", "c5"]:
         log(f"[workloads] {name} (scale {scale})")
         print(json.dumps(RUNNERS[name](scale)), flush=True)
