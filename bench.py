@@ -211,6 +211,8 @@ def main() -> int:
             for name in ("learned", "headline"):
                 if "value" in ab.get(name, {}):
                     ab[f"ratio_{name}_vs_plain"] = ab[name]["value"] / ab["plain"]["value"]
+        # "ordering" = learned: evaluator move keys near the root and an
+        # aspiration window around the evaluator's estimate of each board
         ab["knobs_differing"] = {"learned_vs_plain": ["ordering"],
                                  "headline_vs_plain": ["ordering", "tt_log2_buckets", "tt_min_empties",
                                                        "positions_per_wave"]}

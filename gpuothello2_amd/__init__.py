@@ -85,7 +85,8 @@ class Config(ctypes.Structure):
         ("think_depth", ctypes.c_int32),
         ("leaf_batch_min", ctypes.c_int32),
         ("score_batch_min", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 3),
+        ("aspiration_width", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 

@@ -207,6 +207,9 @@ struct SolveArgs {
                       // them together once this many are queued (0 = each lane on its own)
   int score_batch;    // nodes that order their children queue them and the wavefront computes
                       // the keys together once this many are queued (0 = one child per trip)
+  int asp_width;      // ORDER 2: board roots with >= asp_min empties are searched with the window
+  int asp_min;        // evaluator estimate +- asp_width (discs, even), re-searched once with one
+                      // bound open when the value falls outside (0 = full window)
   int think_depth;    // think mode: moves before the evaluator scores a position
   float ev_scale;     // evaluator units per disc
   // transposition table (null = none): probed when pushing children with
@@ -464,6 +467,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
     }
   };
   bool scoring = false, negate = false;
+  bool asp = false;             // the board's root is searched with an aspiration window
   bool pchk = false;            // a child without moves: decide pass / game over next trip
   int pchk_sq = 0;
   uint64_t pchk_f = 0;
@@ -674,6 +678,24 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
                 Eroot = E;
                 alpha = -64;
                 beta = 64;
+                asp = false;
+                if constexpr (ORDER == 2 && !THINK) {
+                  if (a.asp_width > 0 && E >= a.asp_min) {
+                    // window around the evaluator's estimate of the score
+                    // (side to move, whole discs rounded to an even value)
+                    int ev = a.ev_bias[E];
+                    for_each_feature(P, O, t3, [&](int, uint32_t fi) { ev += a.ev_w[fi]; });
+                    const int g = 2 * (int)floorf((float)ev / a.ev_scale * 0.5f + 0.5f);
+                    alpha = max(-64, g - a.asp_width);
+                    beta = min(64, g + a.asp_width);
+                    if (alpha >= beta || (alpha == -64 && beta == 64)) {
+                      alpha = -64;
+                      beta = 64;
+                    } else {
+                      asp = true;
+                    }
+                  }
+                }
                 owner = -1;
                 mkey = -1;
                 if constexpr (THINK) ebase = E - a.think_depth - 2;
@@ -1329,6 +1351,22 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
           a.tt[h & a.tt_mask] = make_ulonglong2(h ^ d, d);
         }
         if (E == Eroot) {
+          if (asp && owner < 0) {
+            // Aspiration root: a value at or outside the window is only a
+            // bound (fail-soft); the true value lies on its side, so one
+            // search with that side open is exact.
+            asp = false;
+            const int a0 = alpha0_of();
+            const bool low = best <= a0 && a0 > -64, high = best >= beta && beta < 64;
+            if (low || high) {
+              alpha = low ? -64 : best - 2;
+              beta = low ? best + 2 : 64;
+              mkey = -1;
+              mode = SEARCH;
+              enter_node(mobility(P, O), kPV);
+              break;
+            }
+          }
           const int r = negate ? -best : best;
           if (owner < 0) {
             a.scores[idx] = (int8_t)r;
@@ -1668,6 +1706,8 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   args.max_pops = cfg.max_pops_per_trip > 0 ? cfg.max_pops_per_trip : 2;
   args.leaf_batch = sp.think || cfg.leaf_batch_min > 64 ? 0 : (cfg.leaf_batch_min > 0 ? cfg.leaf_batch_min : 48);
   args.score_batch = sp.think || cfg.score_batch_min > 64 ? 0 : (cfg.score_batch_min > 0 ? cfg.score_batch_min : 64);
+  args.asp_width = sp.think || cfg.aspiration_width > 64 ? 0 : 2 * ((cfg.aspiration_width + 1) / 2);
+  args.asp_min = 10;
   if (gp.help) {
     const unsigned int nslots = kHelpSlots;
     const size_t mb = mailbox_bytes(sp, gp);

@@ -98,7 +98,11 @@ typedef struct go2_config {
                                    wavefront computes the queued ordering keys together once
                                    this many (1..64) are waiting; 0 = auto (64), > 64 = each lane
                                    scores its own, one child per loop trip */
-  int32_t reserved[3];
+  int32_t aspiration_width;     /* GO2_ORDER_LEARNED: each board (>= 10 empties) is first searched
+                                   with the window evaluator estimate +- this many discs, and
+                                   searched again with one bound open when its value falls outside
+                                   (scores are exact either way); 0 = full window; default 8 */
+  int32_t reserved[2];
 } go2_config;
 
 /* Counters of one solve.  Replaces the reference's `total` node counter

@@ -18,6 +18,7 @@ static void defaults(go2_config* c) {
   c->learned_min_empties = 11;
   c->block_size = 256;
   c->time_limit_ms = 600000;
+  c->aspiration_width = 8;
 }
 
 int main(int argc, char** argv) {

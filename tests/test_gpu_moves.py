@@ -108,3 +108,21 @@ def test_cli_best_move_column(tmp_path):
             assert (mask >> sq) & 1 and go2.move_text(sq) == mv
         else:
             assert mv == ("ps" if kind == ol.MOVES_PASS else "--")
+
+
+@pytest.mark.parametrize("width", [2, 4, 8, 30])
+def test_aspiration_windows_exact(width):
+    # Learned ordering searches each board first with the window estimate
+    # +- width and once more with one bound open when the value falls outside;
+    # narrow windows force that re-search on most boards.  Scores and best
+    # moves must stay exact.
+    from gpuothello2_amd import learn
+    learn.install()
+    b = np.concatenate([ol.generate(811, 12, 300, 12), ol.generate(812, 12, 120, 14)])
+    want, masks, kinds = ol.best_moves_batch(b, 14, threads=16)
+    cfg = go2.default_config(ordering=go2.ORDER_LEARNED, aspiration_width=width, time_limit_ms=100000)
+    s, m, _ = go2.solve_moves(go2.as_boards(b), 14, cfg)
+    assert np.array_equal(s, want), width
+    for i in range(len(b)):
+        if kinds[i] == ol.MOVES_PLAY:
+            assert (int(masks[i]) >> int(m[i])) & 1, (width, i)
