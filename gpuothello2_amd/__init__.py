@@ -86,7 +86,8 @@ class Config(ctypes.Structure):
         ("leaf_batch_min", ctypes.c_int32),
         ("score_batch_min", ctypes.c_int32),
         ("aspiration_width", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 2),
+        ("pvs_min_empties", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 1),
     ]
 
 

@@ -90,7 +90,8 @@ void apply_json_config(const char* path, go2_config* c) {
                 {"export_low_empties", &c->export_low_empties},
                 {"leaf_batch_min", &c->leaf_batch_min},
                 {"score_batch_min", &c->score_batch_min},
-                {"aspiration_width", &c->aspiration_width}};
+                {"aspiration_width", &c->aspiration_width},
+                {"pvs_min_empties", &c->pvs_min_empties}};
   size_t pos = 0;
   while ((pos = text.find('"', pos)) != std::string::npos) {
     const size_t end = text.find('"', pos + 1);

@@ -56,6 +56,7 @@ int resolve_config(const go2_config* cfg, go2_config* out) {
   if (c.leaf_batch_min < 0) return fail(GO2_E_ARG, "config.leaf_batch_min must be >= 0");
   if (c.score_batch_min < 0) return fail(GO2_E_ARG, "config.score_batch_min must be >= 0");
   if (c.aspiration_width < 0) return fail(GO2_E_ARG, "config.aspiration_width must be >= 0");
+  if (c.pvs_min_empties < 0) return fail(GO2_E_ARG, "config.pvs_min_empties must be >= 0");
   if (c.split_cut_after < 0 || c.split_cut_after > 3)
     return fail(GO2_E_ARG, "config.split_cut_after must be in [0, 3]");
   *out = c;
@@ -157,6 +158,7 @@ void go2_config_default(go2_config* cfg) {
   cfg->lds_levels = 0;
   cfg->time_limit_ms = 600000;
   cfg->aspiration_width = 8;
+  cfg->pvs_min_empties = 8;
 }
 
 // base81: character i encodes squares 4i..4i+3 as 33 + d0 + 3 d1 + 9 d2 + 32 d3,

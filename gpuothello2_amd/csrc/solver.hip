@@ -207,6 +207,8 @@ struct SolveArgs {
                       // them together once this many are queued (0 = each lane on its own)
   int score_batch;    // nodes that order their children queue them and the wavefront computes
                       // the keys together once this many are queued (0 = one child per trip)
+  int pvs_min;        // children after the first of nodes with >= pvs_min empties are searched
+                      // with a null window first (0 = full windows)
   int asp_width;      // ORDER 2: board roots with >= asp_min empties are searched with the window
   int asp_min;        // evaluator estimate +- asp_width (discs, even), re-searched once with one
                       // bound open when the value falls outside (0 = full window)
@@ -468,6 +470,8 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
   };
   bool scoring = false, negate = false;
   bool asp = false;             // the board's root is searched with an aspiration window
+  uint32_t nw_mask = 0;         // bit E-3: the child of frame E is searched with a null window
+  bool full_next = false;       // the next child of the current node is a re-search: full window
   bool pchk = false;            // a child without moves: decide pass / game over next trip
   int pchk_sq = 0;
   uint64_t pchk_f = 0;
@@ -569,6 +573,8 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
                               bool ext, int csq) {
     ++c_nodes;
     mkey = ((ca + 64) << 6) | csq;
+    nw_mask = 0;
+    full_next = false;
     E = oe - 1;
     Eroot = E;
     owner = own;
@@ -698,6 +704,8 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
                 }
                 owner = -1;
                 mkey = -1;
+                nw_mask = 0;
+                full_next = false;
                 if constexpr (THINK) ebase = E - a.think_depth - 2;
                 mode = SEARCH;
                 enter_node(m, kPV);
@@ -1106,6 +1114,21 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
         }
       }
       int ca = passed ? alpha : -beta, cb = passed ? beta : -alpha;  // child window
+      bool nullw = false;
+      if constexpr (!THINK) {
+        // Principal variation search: after the first child, a child is
+        // first searched with the null window (alpha, alpha + 2) (every
+        // value is even); one that fails high inside (alpha, beta) is
+        // searched again with its bound as alpha.
+        if (push) {
+          if (!passed && a.pvs_min && E >= a.pvs_min && ndone > 0 && cb - ca > 2 && !full_next) {
+            ca = cb - 2;
+            nullw = true;
+          }
+          full_next = false;
+        }
+      }
+      bool research = false;
       if (push && !passed && a.tt && E - 1 >= a.tt_min) {
         // Probe the child: an entry may settle it or narrow its window.
         ++c_probes;
@@ -1131,11 +1154,24 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
             ++c_cuts;
             push = false;
             v = -tv;
-            commit = true;
+            // an upper bound below a null window is only a bound on our side
+            if (nullw && tv == up && lo != up && v < beta) {
+              research = true;
+            } else {
+              commit = true;
+            }
           }
         }
       }
+      if (research) {
+        // search this child again with the full window, next
+        moves |= bit;
+        order = ((order << 7) | (uint64_t)(sq + 1)) & kFull63;
+        alpha = max(alpha, v - 2);
+        full_next = true;
+      }
       if (push) {
+        if (nullw) nw_mask |= 1u << (E - 3);
         push_frame(E, f | bit, moves, order, frame_word<THINK>(alpha, beta, best, sq, passed, ndone, ntype));
         // Young-brothers-wait: siblings are offered after the first child
         // (after cut_after children at expected CUT nodes).  When nearly
@@ -1406,14 +1442,30 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
         const int pbest = fw_best<THINK>(w);
         undo_move(P, O, fb, w);
         const int vv = ((w >> 27) & 1) ? best : -best;
-        if (E == Eroot && owner < 0) mkey = max(mkey, move_key(vv, vv > palpha, (int)((w >> 21) & 63)));
+        const int msq = (int)((w >> 21) & 63);
+        bool again = false;
+        if constexpr (!THINK) {
+          if ((nw_mask >> (E - 3)) & 1) {
+            nw_mask &= ~(1u << (E - 3));
+            again = vv > palpha && vv < pbeta;  // a lower bound inside the window
+          }
+        }
         beta = pbeta;
-        best = max(pbest, vv);
-        alpha = max(palpha, best);
+        if (again) {
+          best = pbest;
+          alpha = vv - 2;
+          moves |= 1ULL << msq;
+          order = ((order << 7) | (uint64_t)(msq + 1)) & kFull63;
+          full_next = true;
+        } else {
+          if (E == Eroot && owner < 0) mkey = max(mkey, move_key(vv, vv > palpha, msq));
+          best = max(pbest, vv);
+          alpha = max(palpha, best);
+        }
         if (E >= a.split_min) {
           const uint32_t psx = get_sx(E);
           cur_sx = psx;
-          a0mv = (((sx_alpha0(psx) >> 1) & 127) << 6) | (vv > pbest ? (int)((w >> 21) & 63) : (int)((psx >> 23) & 63));
+          a0mv = (((sx_alpha0(psx) >> 1) & 127) << 6) | (!again && vv > pbest ? msq : (int)((psx >> 23) & 63));
         }
         if constexpr (THINK) {
           ndone = 1;
@@ -1707,6 +1759,7 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   args.leaf_batch = sp.think || cfg.leaf_batch_min > 64 ? 0 : (cfg.leaf_batch_min > 0 ? cfg.leaf_batch_min : 48);
   args.score_batch = sp.think || cfg.score_batch_min > 64 ? 0 : (cfg.score_batch_min > 0 ? cfg.score_batch_min : 64);
   args.asp_width = sp.think || cfg.aspiration_width > 64 ? 0 : 2 * ((cfg.aspiration_width + 1) / 2);
+  args.pvs_min = sp.think || cfg.pvs_min_empties <= 0 ? 0 : max(cfg.pvs_min_empties, 5);
   args.asp_min = 10;
   if (gp.help) {
     const unsigned int nslots = kHelpSlots;

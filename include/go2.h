@@ -102,7 +102,11 @@ typedef struct go2_config {
                                    with the window evaluator estimate +- this many discs, and
                                    searched again with one bound open when its value falls outside
                                    (scores are exact either way); 0 = full window; default 8 */
-  int32_t reserved[2];
+  int32_t pvs_min_empties;      /* principal variation search: after its first child, a node with
+                                   >= this many empties (>= 5) searches each child with a null
+                                   window first and again with a full one when it fails high
+                                   inside the window; 0 = off; default 8 */
+  int32_t reserved[1];
 } go2_config;
 
 /* Counters of one solve.  Replaces the reference's `total` node counter

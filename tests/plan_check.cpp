@@ -19,6 +19,7 @@ static void defaults(go2_config* c) {
   c->block_size = 256;
   c->time_limit_ms = 600000;
   c->aspiration_width = 8;
+  c->pvs_min_empties = 8;
 }
 
 int main(int argc, char** argv) {

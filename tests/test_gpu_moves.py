@@ -126,3 +126,23 @@ def test_aspiration_windows_exact(width):
     for i in range(len(b)):
         if kinds[i] == ol.MOVES_PLAY:
             assert (int(masks[i]) >> int(m[i])) & 1, (width, i)
+
+
+@pytest.mark.parametrize("pvs,tt,ordering", [(5, 0, "static"), (5, 20, "learned"), (8, 22, "learned"),
+                                             (6, 0, "mobility")])
+def test_null_window_search_exact(pvs, tt, ordering):
+    # Principal variation search: later children get a null window first and
+    # a full re-search when they fail high inside the window (also when the
+    # table settles them with a bound), with helpers and the table on.
+    from gpuothello2_amd import learn
+    learn.install()
+    b = np.concatenate([ol.generate(821, 12, 300, 12), ol.generate(822, 12, 120, 14)])
+    want, masks, kinds = ol.best_moves_batch(b, 14, threads=16)
+    order = {"static": go2.ORDER_STATIC, "mobility": go2.ORDER_MOBILITY, "learned": go2.ORDER_LEARNED}[ordering]
+    cfg = go2.default_config(ordering=order, pvs_min_empties=pvs, tt_log2_buckets=tt, tt_min_empties=6,
+                             split_min_empties=6, time_limit_ms=100000)
+    s, m, st = go2.solve_moves(go2.as_boards(b), 14, cfg)
+    assert np.array_equal(s, want), (pvs, tt, ordering)
+    for i in range(len(b)):
+        if kinds[i] == ol.MOVES_PLAY:
+            assert (int(masks[i]) >> int(m[i])) & 1, (pvs, i)
