@@ -53,9 +53,11 @@ def boards_of(fx):
 
 def plain_config(**kw):
     """The plain batched kernel: one lane per board, static order, no table,
-    no intra- or cross-wavefront help."""
+    no intra- or cross-wavefront help, no leaf batches (each lane scores its
+    own leaves)."""
     return go2.default_config(ordering=go2.ORDER_STATIC, positions_per_wave=64, split_min_empties=99,
-                              export_min_empties=99, tt_log2_buckets=0, time_limit_ms=TIME_LIMIT_MS, **kw)
+                              export_min_empties=99, leaf_batch_min=65, tt_log2_buckets=0,
+                              time_limit_ms=TIME_LIMIT_MS, **kw)
 
 
 def solve_task(b, depth, cfg, launches=1):

@@ -83,10 +83,15 @@ def c2(scale: float) -> dict:
     ol = _oracle()
     n = max(1000, int(1000000 * scale))
     b = generate(202, 2, n, 12)
-    s, st, _ = _solve(b, 12, default_config(ordering=ORDER_STATIC), "c2")
+    # the plain batched kernel: one lane per board, static order, no table, no help, no leaf batches
+    plain = default_config(ordering=ORDER_STATIC, positions_per_wave=64, split_min_empties=99,
+                           export_min_empties=99, leaf_batch_min=65)
+    s, st, _ = _solve(b, 12, plain, "c2")
     k = min(n, 20000)
     cpu, _, _ = ol.solve_batch(_pairs(b[:k]), 12, threads=min(16, os.cpu_count() or 1))
+    d, std, _ = _solve(b, 12, default_config(ordering=ORDER_STATIC), "c2 default shape")
     return {"config": "C2", "positions": n, "empties": 12, "ordering": "static",
+            "default_shape_positions_per_s": n / (std["kernel_ms"] / 1e3), "default_shape_match": bool(np.array_equal(d, s)),
             "positions_per_s": n / (st["kernel_ms"] / 1e3), "nodes_per_position": st["nodes"] / n,
             "checked_vs_cpu": k, "match": bool(np.array_equal(cpu, s[:k]))}
 
