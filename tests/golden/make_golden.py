@@ -6,10 +6,9 @@ Run in the build container (where the reference sources exist) after
     python tests/golden/make_golden.py
 
 Fixtures:
-  ref_bitboard.npz  flip / mobility / final_score / base81 text computed by the
-                    reference's own host functions (board.cu, to_board.cpp,
-                    compiled from the reference sources by oracle/Makefile) on
-                    random and playout boards.
+  ref_bitboard.npz  base81 text of random and playout boards written by the
+                    reference's own fromBoard (to_board.cpp, compiled unchanged
+                    from the reference sources by oracle/Makefile).
   solve_small.npz   exact scores of boards with 0..10 empties: the oracle's
                     reference-method search, asserted equal to a no-pruning
                     minimax when written.
@@ -74,28 +73,19 @@ def main():
     R = ol.reference()
     rng = np.random.default_rng(20261203)
 
-    # ---- reference bitboard vectors ---------------------------------------------
+    # ---- reference base81 vectors -----------------------------------------------
     boards = random_boards(rng, 400)
     for e in (4, 10, 16, 24, 36, 50):
         boards += [tuple(int(v) for v in b) for b in ol.generate(77, 9, 50, e)]
     n = len(boards)
     P = np.array([b[0] for b in boards], dtype=np.uint64)
     O = np.array([b[1] for b in boards], dtype=np.uint64)
-    flips = np.zeros((n, 64), dtype=np.uint64)
-    mob = np.zeros(n, dtype=np.uint64)
-    fs = np.zeros(n, dtype=np.int8)
     text = []
     buf = ctypes.create_string_buffer(17)
-    for i, (p, o) in enumerate(boards):
-        for sq in range(64):
-            if not ((p | o) >> sq) & 1:
-                flips[i, sq] = R.ref_flip(p, o, sq)
-        mob[i] = R.ref_mobility(p, o)
-        fs[i] = R.ref_final_score(p, o)
+    for p, o in boards:
         R.ref_from_board(p, o, buf)
         text.append(buf.raw[:16].decode("latin-1"))
-    np.savez_compressed(os.path.join(HERE, "ref_bitboard.npz"), player=P, opponent=O, flip=flips,
-                        mobility=mob, final_score=fs, text=np.array(text))
+    np.savez_compressed(os.path.join(HERE, "ref_bitboard.npz"), player=P, opponent=O, text=np.array(text))
 
     # ---- exact scores, small -------------------------------------------------------
     sb = []

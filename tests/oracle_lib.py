@@ -85,26 +85,16 @@ def ref_available() -> bool:
 
 
 def reference() -> ctypes.CDLL:
-    """The reference's own host flip/mobility/codec (built only where its sources are)."""
+    """The reference's own host codec and PEXT (built only where its sources are)."""
     global _ref
     if _ref is None:
         L = ctypes.CDLL(REF_PATH)
-        L.ref_flip.argtypes = [u64, u64, i32]
-        L.ref_flip.restype = u64
-        L.ref_mobility.argtypes = [u64, u64]
-        L.ref_mobility.restype = u64
-        L.ref_mobility_count.argtypes = [u64, u64]
-        L.ref_final_score.argtypes = [u64, u64]
         L.ref_pext.argtypes = [u64, u64]
         L.ref_pext.restype = u64
         L.ref_to_board.argtypes = [ctypes.c_char_p, P64, P64]
         L.ref_to_board.restype = None
         L.ref_from_board.argtypes = [u64, u64, ctypes.c_char_p]
         L.ref_from_board.restype = None
-        L.ref_iteration_order.argtypes = [u64, u64, ctypes.POINTER(ctypes.c_int)]
-        L.ref_move.argtypes = [u64, u64, i32, P64, P64]
-        L.ref_move.restype = None
-        L.ref_mg_score.argtypes = [u64, u64]
         _ref = L
     return _ref
 
