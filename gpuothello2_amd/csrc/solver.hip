@@ -45,8 +45,9 @@
 //     fraction grows); idle wavefronts claim them and report through
 //     per-frame mailboxes; owners take back unclaimed siblings.  Wavefronts
 //     count themselves active when they start and leave once none is.
-//   * Optional lock-free transposition table (Zobrist keys, XOR-checked
-//     16-byte buckets, launch generations), probed when children are pushed.
+//   * Optional lock-free transposition table (64-bit multiply-xorshift keys,
+//     XOR-checked 16-byte buckets, launch generations), probed when children
+//     are pushed.
 //   * Best moves: values reaching a board's root carry the move and an
 //     "exact" flag; the largest such key names an optimal move.
 //   * THINK instantiation: the reference's think mode, a depth-limited
@@ -220,7 +221,6 @@ struct SolveArgs {
   unsigned long long tt_mask;
   int tt_min;
   const unsigned int* tt_gen;  // launch generation (device, set by the reset kernel)
-  const uint64_t* zob;  // Zobrist keys per (byte position 0..15, byte value)
   // cross-wavefront help (null = off): while some wavefront is idle, busy
   // wavefronts export siblings to `slots`: of nodes with >= ext_min empties
   // when few wavefronts are idle, down to ext_lo empties when nearly all
@@ -265,19 +265,26 @@ __device__ __forceinline__ uint32_t sx_init(int alpha0) { return ((uint32_t)(alp
 __device__ __forceinline__ int sx_alpha0(uint32_t x) { return ((int)(x << 9) >> 25) * 2; }
 
 // ---- transposition table ------------------------------------------------------
-// Key: Zobrist hashing over the 16 bytes of the two bitboards (one random
-// 64-bit word per byte position and value).  Bucket: (key ^ data, data), so a
-// torn or foreign entry fails the key check (lock-free, no CAS).  Data:
-// lower + 64 (bits 0-7), upper + 64 (8-15), best move (16-21, 63 = none),
-// generation (24-31), empties (32-37).
-__device__ __forceinline__ uint64_t zobrist(uint64_t P, uint64_t O, const uint64_t* z) {
-  uint64_t h = 0;
-#pragma unroll
-  for (int b = 0; b < 8; ++b) {
-    h ^= z[b * 256 + (int)((P >> (8 * b)) & 255)];
-    h ^= z[(8 + b) * 256 + (int)((O >> (8 * b)) & 255)];
-  }
-  return h;
+// Key: a 64-bit hash of both bitboards, K(P, O) = fmix64(P ^ fmix64(O ^ c)),
+// fmix64 being the bijective finalizer of MurmurHash3 (two multiply-xorshift
+// rounds).  It costs four 64-bit multiplies and no memory access, where
+// tabulated Zobrist keys (one random word per byte position and value) cost
+// 16 dependent cache loads per key.  For a fixed O it is a bijection of P,
+// and positions with different O differ by fmix64 outputs, so two positions
+// of one search share a key with probability ~2^-64, as with Zobrist keys.
+// Bucket: (key ^ data, data), so a torn or foreign entry fails the key check
+// (lock-free, no CAS).  Data: lower + 64 (bits 0-7), upper + 64 (8-15), best
+// move (16-21, 63 = none), generation (24-31), empties (32-37).
+__device__ __forceinline__ uint64_t fmix64(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xFF51AFD7ED558CCDULL;
+  x ^= x >> 33;
+  x *= 0xC4CEB9FE1A85EC53ULL;
+  x ^= x >> 33;
+  return x;
+}
+__device__ __forceinline__ uint64_t tt_key(uint64_t P, uint64_t O) {
+  return fmix64(P ^ fmix64(O ^ 0x9E3779B97F4A7C15ULL));
 }
 
 __device__ __forceinline__ uint64_t readlane64(uint64_t x, int lane) {
@@ -1132,7 +1139,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
       if (push && !passed && a.tt && E - 1 >= a.tt_min) {
         // Probe the child: an entry may settle it or narrow its window.
         ++c_probes;
-        const uint64_t h = zobrist(cP, cO, a.zob);
+        const uint64_t h = tt_key(cP, cO);
         const ulonglong2 b = a.tt[h & a.tt_mask];
         if ((b.x ^ b.y) == h && (uint32_t)((b.y >> 24) & 255) == tt_gen) {
           ++c_hits;
@@ -1383,7 +1390,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
           const int lo = best > alpha0_of() ? best : -64, up = best < beta ? best : 64;
           const uint64_t d = (uint64_t)(lo + 64) | ((uint64_t)(up + 64) << 8) | ((uint64_t)bestmv_of() << 16) |
                              ((uint64_t)tt_gen << 24) | ((uint64_t)E << 32);
-          const uint64_t h = zobrist(P, O, a.zob);
+          const uint64_t h = tt_key(P, O);
           a.tt[h & a.tt_mask] = make_ulonglong2(h ^ d, d);
         }
         if (E == Eroot) {
@@ -1529,8 +1536,8 @@ struct EvalTables {
   float scale = 0.f;
 };
 static EvalTables g_eval[64];
-// Guards the per-device evaluator tables and Zobrist keys while they are set up.
-static std::mutex g_zob_mutex;
+// Guards the per-device evaluator tables while they are set up.
+static std::mutex g_eval_mutex;
 
 }  // namespace go2
 
@@ -1541,7 +1548,7 @@ extern "C" int go2_set_evaluator(const int16_t* w, const int16_t* bias, float sc
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
   if (dev < 0 || dev >= 64) return fail(GO2_E_ARG, "device index out of range");
-  std::lock_guard<std::mutex> lock(g_zob_mutex);
+  std::lock_guard<std::mutex> lock(g_eval_mutex);
   EvalTables& t = g_eval[dev];
   if (!t.w && (e = hipMalloc(&t.w, kPatWeights * sizeof(int16_t))) != hipSuccess)
     return hip_fail(e, "hipMalloc(evaluator)");
@@ -1632,37 +1639,6 @@ int Workspace::ensure_tt(int log2_buckets, hipStream_t stream) {
     hipLaunchKernelGGL(clear_tt_kernel, dim3(4096), dim3(256), 0, stream, (ulonglong2*)tt, n);
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "clear table launch");
   }
-  return GO2_OK;
-}
-
-// Zobrist keys per device: 16 byte positions x 256 byte values, SplitMix64.
-static uint64_t* g_zob[64];
-
-// The keys of one device, built once; the pointer is published only after
-// the keys are on the device, under a lock (tasks may launch from several
-// host threads).
-static int zobrist_table(int dev, const uint64_t** out) {
-  std::lock_guard<std::mutex> lock(g_zob_mutex);
-  if (!g_zob[dev]) {
-    uint64_t h[16 * 256];
-    uint64_t st = 0x5EED0F2A11CE5EEDULL;
-    for (int i = 0; i < 16 * 256; ++i) {
-      st += 0x9E3779B97F4A7C15ULL;
-      uint64_t z = st;
-      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-      z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-      h[i] = z ^ (z >> 31);
-    }
-    hipError_t e;
-    uint64_t* d = nullptr;
-    if ((e = hipMalloc(&d, sizeof(h))) != hipSuccess) return hip_fail(e, "hipMalloc(zobrist)");
-    if ((e = hipMemcpy(d, h, sizeof(h), hipMemcpyHostToDevice)) != hipSuccess) {
-      (void)hipFree(d);
-      return hip_fail(e, "hipMemcpy(zobrist)");
-    }
-    g_zob[dev] = d;
-  }
-  *out = g_zob[dev];
   return GO2_OK;
 }
 
@@ -1782,7 +1758,6 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   }
   if (cfg.tt_log2_buckets > 0 && !sp.think) {
     if ((rc = ws.ensure_tt(cfg.tt_log2_buckets, stream)) != GO2_OK) return rc;
-    if ((rc = zobrist_table(dev, &args.zob)) != GO2_OK) return rc;
     args.tt = reinterpret_cast<ulonglong2*>(ws.tt);
     args.tt_mask = (1ULL << cfg.tt_log2_buckets) - 1;
     args.tt_gen = &ws.ctr->tt_gen;

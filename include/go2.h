@@ -77,7 +77,8 @@ typedef struct go2_config {
   int32_t learned_key_mobility;
   int32_t max_pops_per_trip;    /* finished nodes a lane returns from per loop trip; 0 = auto (2) */
   int32_t tt_log2_buckets;      /* transposition table of 2^n 16-byte buckets shared by the whole
-                                   launch (Zobrist keys, lock-free); 0 = no table */
+                                   launch (64-bit keys hashed from both bitboards, lock-free);
+                                   0 = no table */
   int32_t tt_min_empties;       /* probe/store nodes with >= this many empties; 0 = auto (12) */
   int32_t graph_launch;         /* tasks capture their launch (counter reset + solve kernel) in a
                                    HIP graph once and replay it on later launches */
