@@ -1342,16 +1342,35 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
             key = move_key_of(cP, cO, mobility(cP, cO), bit, jE);
             ++c_steps;
           }
-          int rank = 0;
-          for (int j = 0; __ballot(worker && j < jcnt); ++j) {
-            const int kj = __shfl(key, min(joff + j, 63));
-            if (worker && j < jcnt && (kj < key || (kj == key && j < k))) ++rank;
+          // Rank of each job inside its node's segment of lanes [joff,
+          // joff + jcnt): jobs with a smaller key, then equal keys on lower
+          // lanes (lower squares).  Keys are 6-bit; six ballots from the top
+          // bit down split the lanes into "key below mine" and "key equal".
+          uint64_t lt = 0, eq = ~0ull;
+#pragma unroll
+          for (int b = 5; b >= 0; --b) {
+            const uint64_t B = __ballot((key >> b) & 1);
+            const bool one = (key >> b) & 1;
+            lt |= one ? (eq & ~B) : 0ull;
+            eq &= one ? B : ~B;
           }
-          const uint64_t contrib = worker && rank < 9 ? (uint64_t)(jsq + 1) << (7 * rank) : 0;
+          const uint64_t seg = (jcnt >= 64 ? ~0ull : ((1ull << jcnt) - 1)) << (joff & 63);
+          const int rank = popc64(lt & seg) + popc64(eq & seg & __lanemask_lt());
+          // Each node's first nine squares in rank order, through the same
+          // LDS map (every lane has read its entry above): job slot joff +
+          // rank gets square + 1, the node's lane reads its slots back.
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          if (worker && rank < 9) lmap[joff + rank] = (uint8_t)(jsq + 1);
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
           uint64_t ord = 0;
-          for (int j = 0; __ballot(sel && j < cnt); ++j) {
-            const uint64_t c = shfl64(contrib, min(off + j, 63));
-            if (sel && j < cnt) ord |= c;
+          if (sel) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i)
+              if (i < cnt) ord |= (uint64_t)lmap[min(off + i, 63)] << (7 * i);
           }
           if (sel) {
             order = ord;
