@@ -54,7 +54,9 @@ GO2_HD uint64_t flip_line(uint64_t P, uint64_t Om, uint64_t line, uint64_t above
   uint64_t f = (out_up - (uint64_t)(out_up != 0)) & up;
   uint64_t dn = line & below;
   uint64_t out_dn = high_bit(dn & ~Om) & P;
-  f |= (0 - (out_dn << 1)) & dn;
+  // squares of dn above out_dn (none when out_dn is 0).  Written without
+  // -(out_dn << 1), which the compiler turns into a 64-bit multiply by -2.
+  f |= dn & ~(out_dn | (out_dn - 1));
   return f;
 }
 
@@ -209,6 +211,30 @@ GO2_HD int solve2_pass(uint64_t P, uint64_t O, int a, int b, const T& t,
   nodes += 1 + (ga != 0) + (gb != 0);
   if (best_sq) *best_sq = 64;
   return -(wa > wb ? wa : wb);
+}
+
+// The score of solve2 when P has a legal move on one of the two empty
+// squares; kNoMove2 when P has none (the caller settles the pass itself, so
+// that many lanes running this together never run the pass branch).
+constexpr int kNoMove2 = 127;
+template <class T>
+GO2_HD int solve2_mover(uint64_t P, uint64_t O, const T& t, int& nodes) {
+  const uint64_t e = ~(P | O);
+  const int a = ctz64(e);
+  const int b = 63 - __builtin_clzll(e & (e - 1));
+  int best = -65;
+  uint64_t f = flip(P, O, a);
+  if (f) {
+    best = -last_score_tab(O ^ f, P ^ f ^ (1ULL << a), b, t);
+    ++nodes;
+  }
+  f = flip(P, O, b);
+  if (f) {
+    const int v = -last_score_tab(O ^ f, P ^ f ^ (1ULL << b), a, t);
+    ++nodes;
+    best = v > best ? v : best;
+  }
+  return best > -65 ? best : kNoMove2;
 }
 
 // Exact score for P to move with exactly two empty squares, in straight-line

@@ -330,13 +330,22 @@ __device__ __forceinline__ uint64_t shfl64(uint64_t x, int src) {
 
 // A queued child of a node with three empties: play sq on (P, O) and score
 // the resulting two-empties position exactly; returns the value for the
-// node's side (the negated child score).
+// node's side.  The child's mover may have no move there: the first round
+// (swap false) then returns kNeedPass and the node queues the same move
+// again with swap true, which scores the child with the node's side to move
+// (or the final count when neither side can move).  A whole batch of lanes
+// thus never runs a pass branch that only a few of them need.
+constexpr int kNeedPass = 126;
 template <class T>
-__device__ __forceinline__ int leaf_job(uint64_t P, uint64_t O, int sq, const T& lastc, int& nodes) {
+__device__ __forceinline__ int leaf_job(uint64_t P, uint64_t O, int sq, bool swap, const T& lastc, int& nodes) {
   const uint64_t f = flip(P, O, sq);
   const uint64_t bit = 1ULL << sq;
-  ++nodes;
-  return -solve2(O ^ f, P ^ f ^ bit, lastc, nodes);
+  const uint64_t cP = swap ? P ^ f ^ bit : O ^ f;  // side to move in the scored position
+  const uint64_t cO = swap ? O ^ f : P ^ f ^ bit;
+  ++nodes;  // the child, or (swap) its pass
+  const int r = solve2_mover(cP, cO, lastc, nodes);
+  if (r != kNoMove2) return swap ? r : -r;
+  return swap ? final_score(cP, cO) : kNeedPass;
 }
 
 template <class T>
@@ -480,6 +489,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
   uint32_t nw_mask = 0;         // bit E-3: the child of frame E is searched with a null window
   bool full_next = false;       // the next child of the current node is a re-search: full window
   bool pchk = false;            // a child without moves: decide pass / game over next trip
+  bool leaf_swap = false;       // LEAFWAIT: the queued children are passes (see leaf_job)
   int pchk_sq = 0;
   uint64_t pchk_f = 0;
   bool pop_pending = false;     // more finished nodes to return from next trip
@@ -570,6 +580,7 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
       // every child is queued at once and scored exactly (an exact value is
       // a valid fail-soft result whatever the window)
       mode = LEAFWAIT;
+      leaf_swap = false;
       scoring = false;
     }
   };
@@ -1262,12 +1273,13 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
           const int r = lane < njobs ? (int)lmap[lane] : lane;
           const uint64_t jP = shfl64(P, r), jO = shfl64(O, r), jm = shfl64(moves, r);
           const int k = lane - __shfl(off, r);
+          const bool jswap = __shfl((int)leaf_swap, r) != 0;
           int jv = 0;
           if (lane < njobs) {
             uint64_t m = jm;
             for (int i = 0; i < k; ++i) m &= m - 1;
             int nl = 0;
-            jv = leaf_job(jP, jO, ctz64(m), lastc, nl);
+            jv = leaf_job(jP, jO, ctz64(m), jswap, lastc, nl);
             c_nodes += (unsigned int)nl;
           }
           // results back to the requesters
@@ -1275,23 +1287,34 @@ __global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
 #pragma unroll
           for (int i = 0; i < 3; ++i) vk[i] = __shfl(jv, min(off + i, 63));
           if (sel) {
-            uint64_t m = moves;
-            int nb = -64;
+            uint64_t m = moves, pend = 0;
+            int nb = -64, got = 0;
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
               if (i < cnt) {
                 const int sq = ctz64(m);
                 m &= m - 1;
-                if (E == Eroot && owner < 0) mkey = max(mkey, move_key(vk[i], true, sq));
-                nb = max(nb, vk[i]);
+                if (vk[i] == kNeedPass) {
+                  pend |= 1ULL << sq;
+                } else {
+                  if (E == Eroot && owner < 0) mkey = max(mkey, move_key(vk[i], true, sq));
+                  nb = max(nb, vk[i]);
+                  ++got;
+                }
               }
             }
             best = max(best, nb);
             alpha = max(alpha, best);
-            ndone = min(ndone + cnt, 3);
-            moves = 0;
-            mode = SEARCH;
-            commit = true;
+            ndone = min(ndone + got, 3);
+            if (pend && alpha < beta) {
+              // children whose mover passes: queued again for the next batch
+              moves = pend;
+              leaf_swap = true;
+            } else {
+              moves = 0;
+              mode = SEARCH;
+              commit = true;
+            }
           }
         }
       }
