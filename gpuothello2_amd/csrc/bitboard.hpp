@@ -43,20 +43,54 @@ GO2_HD uint64_t high_bit(uint64_t x) {
   return (0x8000000000000000ULL >> __builtin_clzll(x | 1)) & x;
 }
 
-// Discs flipped along one line through sq.  `line` holds the squares of that
-// line (sq included or not).  Above sq the first non-opponent square is the
-// lowest candidate bit; below sq it is the highest one.  A player disc there
-// closes the run and every square between flips.
-GO2_HD uint64_t flip_line(uint64_t P, uint64_t Om, uint64_t line, uint64_t above, uint64_t below) {
-  uint64_t up = line & above;
-  uint64_t stop_up = up & ~Om;
-  uint64_t out_up = stop_up & (0 - stop_up) & P;
-  uint64_t f = (out_up - (uint64_t)(out_up != 0)) & up;
-  uint64_t dn = line & below;
-  uint64_t out_dn = high_bit(dn & ~Om) & P;
-  // squares of dn above out_dn (none when out_dn is 0).  Written without
-  // -(out_dn << 1), which the compiler turns into a 64-bit multiply by -2.
-  f |= dn & ~(out_dn | (out_dn - 1));
+// Any boolean function of three words in one gfx950 instruction
+// (v_bitop3_b32) per 32-bit half.  TT is the truth table: bit i is the
+// result for the input bits a = i >> 2, b = (i >> 1) & 1, c = i & 1 (so
+// a & b & c is 0x80, a | (b & c) is 0xF8).  Host builds evaluate the table.
+template <int TT>
+GO2_HD uint32_t bop3(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_bitop3_b32(a, b, c, TT);
+#else
+  uint32_t r = 0;
+  for (int i = 0; i < 8; ++i)
+    if ((TT >> i) & 1) r |= ((i & 4) ? a : ~a) & ((i & 2) ? b : ~b) & ((i & 1) ? c : ~c);
+  return r;
+#endif
+}
+template <int TT>
+GO2_HD uint64_t bop3(uint64_t a, uint64_t b, uint64_t c) {
+  const uint32_t hi = bop3<TT>((uint32_t)(a >> 32), (uint32_t)(b >> 32), (uint32_t)(c >> 32));
+  const uint32_t lo = bop3<TT>((uint32_t)a, (uint32_t)b, (uint32_t)c);
+  uint64_t r = ((uint64_t)hi << 32) | lo;
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm("" : "+v"(r));  // one 64-bit register pair, so later 64-bit shifts stay whole
+#endif
+  return r;
+}
+constexpr int kAnd3 = 0x80;      // a & b & c
+constexpr int kAndAndNot = 0x40; // a & b & ~c
+constexpr int kOrAnd = 0xF8;     // a | (b & c)
+constexpr int kOrAndNot = 0xF4;  // a | (b & ~c)
+constexpr int kAndNotOr = 0x10;  // a & ~(b | c)
+
+// Discs flipped along one line through sq.  `up` and `dn` hold the squares
+// of that line above and below sq.  Above sq the first non-opponent square
+// is the lowest candidate bit; below sq it is the highest one.  A player
+// disc there closes the run and every square between flips.
+GO2_HD uint64_t flip_line(uint64_t P, uint64_t Om, uint64_t up, uint64_t dn) {
+  const uint64_t stop_up = up & ~Om;
+  const uint64_t out_up = bop3<kAnd3>(stop_up, 0 - stop_up, P);
+  // squares of up below out_up: out_up - 1, cleared when out_up is 0 (then
+  // out_up - 1 is all ones, the only case with its top bit set)
+  const uint64_t t = out_up - 1;
+  const uint64_t none = (uint64_t)(int64_t)((int32_t)(uint32_t)(t >> 32) >> 31);
+  uint64_t f = bop3<kAndAndNot>(t, up, none);
+  // highest non-opponent square below sq, kept when it is a player disc
+  const uint64_t sd = dn & ~Om;
+  const uint64_t out_dn = bop3<kAnd3>(0x8000000000000000ULL >> __builtin_clzll(sd | 1), sd, P);
+  // squares of dn above out_dn (none when out_dn is 0)
+  f = bop3<kOrAndNot>(f, dn, out_dn | (out_dn - 1));
   return f;
 }
 
@@ -68,37 +102,60 @@ GO2_HD uint64_t flip(uint64_t P, uint64_t O, int sq) {
   const uint64_t above = 0xFFFFFFFFFFFFFFFEULL << sq;
   const uint64_t below = ~(0xFFFFFFFFFFFFFFFFULL << sq);
   const int r = sq >> 3, c = sq & 7;
-  uint64_t f = flip_line(P, O, 0xFFULL << (sq & 56), above, below);
-  f |= flip_line(P, O, 0x0101010101010101ULL << c, above, below);
-  const int d = c - r;  // a1-h8 direction
-  const uint64_t diag = d >= 0 ? 0x8040201008040201ULL >> (8 * d) : 0x8040201008040201ULL << (-8 * d);
-  f |= flip_line(P, O, diag, above, below);
-  const int a = c + r - 7;  // h1-a8 direction
-  const uint64_t anti = a >= 0 ? 0x0102040810204080ULL << (8 * a) : 0x0102040810204080ULL >> (-8 * a);
-  f |= flip_line(P, O, anti, above, below);
+  const uint64_t row = 0xFFULL << (sq & 56);
+  const uint64_t col = 0x0101010101010101ULL << c;
+  // the diagonals through sq, shifted by whole rows (no wrap across an edge):
+  // right shift by max(8d, 0) then left shift by max(-8d, 0), no branch
+  const int d = 8 * (c - r);      // a1-h8 direction
+  const uint64_t diag = (0x8040201008040201ULL >> (d > 0 ? d : 0)) << (d < 0 ? -d : 0);
+  const int a = 8 * (c + r - 7);  // h1-a8 direction
+  const uint64_t anti = (0x0102040810204080ULL << (a > 0 ? a : 0)) >> (a < 0 ? -a : 0);
+  uint64_t f = flip_line(P, O, row & above, row & below);
+  f |= flip_line(P, O, col & above, col & below);
+  f |= flip_line(P, O, diag & above, diag & below);
+  f |= flip_line(P, O, anti & above, anti & below);
   return f;
 }
 
+// A 64-bit shift kept as one 64-bit value: without the (empty) operand
+// barrier the compiler splits a shift whose halves feed separate 32-bit
+// operations into a funnel shift per half (two instructions instead of one).
+GO2_HD uint64_t shl_whole(uint64_t x, int s) {
+  uint64_t y = x << s;
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm("" : "+v"(y));
+#endif
+  return y;
+}
+GO2_HD uint64_t shr_whole(uint64_t x, int s) {
+  uint64_t y = x >> s;
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm("" : "+v"(y));
+#endif
+  return y;
+}
+
 // Legal moves in one direction pair (shift s left and right), filling the
-// opponent run by doubling: 1, 2, then 4 and 6 squares.
+// opponent run by doubling: 1, 2, then 4 and 6 squares.  Each "x |= m & y"
+// step is one bitop3 per 32-bit half.
 GO2_HD uint64_t mob_dir(uint64_t P, uint64_t Om, int s) {
-  uint64_t l = Om & (P << s);
-  uint64_t r = Om & (P >> s);
-  l |= Om & (l << s);
-  r |= Om & (r >> s);
-  const uint64_t pl = Om & (Om << s);
-  const uint64_t pr = pl >> s;
-  l |= pl & (l << (2 * s));
-  r |= pr & (r >> (2 * s));
-  l |= pl & (l << (2 * s));
-  r |= pr & (r >> (2 * s));
-  return (l << s) | (r >> s);
+  uint64_t l = Om & shl_whole(P, s);
+  uint64_t r = Om & shr_whole(P, s);
+  l = bop3<kOrAnd>(l, Om, shl_whole(l, s));
+  r = bop3<kOrAnd>(r, Om, shr_whole(r, s));
+  const uint64_t pl = Om & shl_whole(Om, s);
+  const uint64_t pr = shr_whole(pl, s);
+  l = bop3<kOrAnd>(l, pl, shl_whole(l, 2 * s));
+  r = bop3<kOrAnd>(r, pr, shr_whole(r, 2 * s));
+  l = bop3<kOrAnd>(l, pl, shl_whole(l, 2 * s));
+  r = bop3<kOrAnd>(r, pr, shr_whole(r, 2 * s));
+  return shl_whole(l, s) | shr_whole(r, s);
 }
 
 GO2_HD uint64_t mobility(uint64_t P, uint64_t O) {
   const uint64_t Oi = O & kInner;
-  uint64_t m = mob_dir(P, Oi, 1) | mob_dir(P, O, 8) | mob_dir(P, Oi, 7) | mob_dir(P, Oi, 9);
-  return m & ~(P | O);
+  const uint64_t m = mob_dir(P, Oi, 1) | mob_dir(P, O, 8) | mob_dir(P, Oi, 7) | mob_dir(P, Oi, 9);
+  return bop3<kAndNotOr>(m, P, O);
 }
 
 // Exact score of a board with exactly one empty square `sq`, from P's view:
@@ -159,9 +216,9 @@ struct LinePatterns {
 
 GO2_HD LinePatterns line_patterns(uint64_t x, int sq) {
   const int r = sq >> 3, c = sq & 7;
-  const int d = c - r, a = c + r - 7;
-  const uint64_t dm = d >= 0 ? 0x8040201008040201ULL >> (8 * d) : 0x8040201008040201ULL << (-8 * d);
-  const uint64_t am = a >= 0 ? 0x0102040810204080ULL << (8 * a) : 0x0102040810204080ULL >> (-8 * a);
+  const int d = 8 * (c - r), a = 8 * (c + r - 7);
+  const uint64_t dm = (0x8040201008040201ULL >> (d > 0 ? d : 0)) << (d < 0 ? -d : 0);
+  const uint64_t am = (0x0102040810204080ULL << (a > 0 ? a : 0)) >> (a < 0 ? -a : 0);
   auto fold = [](uint64_t y) {
     y |= y >> 32;
     y |= y >> 16;
@@ -184,14 +241,13 @@ template <class T>
 GO2_HD int last_score_tab(uint64_t P, uint64_t O, int sq, const T& t) {
   const int r = sq >> 3, c = sq & 7;
   const LinePatterns L = line_patterns(P, sq);
-  const int n1 = t[c * 256 + L.row] + t[r * 256 + L.col] + t[c * 256 + L.diag] + t[c * 256 + L.anti];
-  const int p = popc64(P);
-  if (n1) return 2 * (p + n1) + 2 - 64;
+  // both sides' counts in one round of table reads (no branch between them)
   const uint32_t hole = ~(1u << c) & 0xFF, holer = ~(1u << r) & 0xFF;
+  const int n1 = t[c * 256 + L.row] + t[r * 256 + L.col] + t[c * 256 + L.diag] + t[c * 256 + L.anti];
   const int n2 = t[c * 256 + (~L.row & hole)] + t[r * 256 + (~L.col & holer)] +
                  t[c * 256 + (~L.diag & L.diag_on & hole)] + t[c * 256 + (~L.anti & L.anti_on & hole)];
-  if (n2) return 2 * (p - n2) - 64;
-  return final_score(P, O);
+  const int p = popc64(P);
+  return n1 ? 2 * (p + n1) + 2 - 64 : n2 ? 2 * (p - n2) - 64 : final_score(P, O);
 }
 
 // Both moves of O after P, with two empty squares a < b, has passed (rare;
