@@ -20,7 +20,11 @@ constexpr int kFrameBase = 4;
 constexpr size_t kLastTableBytes = 2048;   // last-square flip counts (LDS)
 constexpr size_t kBase3TableBytes = 2048;  // base-3 table of the evaluator (LDS)
 constexpr size_t kLeafMapBytesPerWave = 64; // leaf batch: worker slot -> requesting lane (LDS)
-constexpr int kDefaultLdsLevels = 6;
+// The solve kernel is built for two wavefronts per SIMD (its launch bound:
+// 256 VGPRs, no spills), so one CU holds 8 wavefronts and each workgroup
+// may use its share of the CU's 160 KB of LDS for stack levels.
+constexpr int kWavesPerSimd = 2;
+constexpr size_t kLdsBytesPerCU = 160 * 1024;
 constexpr size_t kFrameBytesPerLevel = 32;  // fb, mv, od (u64) + wd, sx (u32) per lane
 constexpr unsigned int kHelpSlots = 4096;
 constexpr size_t kMaxHelpLanes = size_t(1) << 19;  // slot tags hold global lane ids below 2^19
@@ -54,11 +58,20 @@ inline StackPlan plan_stack(int max_depth, const go2_config& cfg) {
   }
   s.levels = s.maxe + 1 - kFrameBase;
   if (s.levels < 1) s.levels = 1;
-  s.lds_levels = cfg.lds_levels > 0 ? cfg.lds_levels : kDefaultLdsLevels;
+  const size_t bs = (size_t)cfg.block_size;
+  const size_t fixed = kLastTableBytes + (cfg.ordering == GO2_ORDER_LEARNED || s.think ? kBase3TableBytes : 0) +
+                       bs / 64 * kLeafMapBytesPerWave;
+  if (cfg.lds_levels > 0) {
+    s.lds_levels = cfg.lds_levels;
+  } else {
+    // as many levels as fit this workgroup's share of the CU's LDS
+    const size_t blocks_per_cu = (size_t)(4 * kWavesPerSimd) / (bs / 64);
+    const size_t share = kLdsBytesPerCU / (blocks_per_cu ? blocks_per_cu : 1);
+    const size_t fit = share > fixed ? (share - fixed) / (bs * kFrameBytesPerLevel) : 0;
+    s.lds_levels = fit < 1 ? 1 : (int)fit;
+  }
   if (s.lds_levels > s.levels) s.lds_levels = s.levels;
-  s.lds_bytes = (size_t)s.lds_levels * (size_t)cfg.block_size * kFrameBytesPerLevel + kLastTableBytes +
-                (cfg.ordering == GO2_ORDER_LEARNED || s.think ? kBase3TableBytes : 0) +
-                (size_t)cfg.block_size / 64 * kLeafMapBytesPerWave;
+  s.lds_bytes = (size_t)s.lds_levels * bs * kFrameBytesPerLevel + fixed;
   return s;
 }
 

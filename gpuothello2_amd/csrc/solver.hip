@@ -384,7 +384,11 @@ __device__ __forceinline__ void undo_move(uint64_t& P, uint64_t& O, uint64_t fb,
 // per board (no helpers, no table: stored values would depend on each
 // board's horizon), learned ordering.
 template <int ORDER, bool THINK = false>
-__global__ void __launch_bounds__(256, 3) solve_kernel(SolveArgs a) {
+// Two wavefronts per SIMD: 256 VGPRs hold the whole loop state without
+// spills, and the two workgroups of 256 lanes per CU keep nine stack levels
+// (4..12) in LDS; three per SIMD (168 VGPRs, spills, six LDS levels) ran no
+// faster.
+__global__ void __launch_bounds__(256, 2) solve_kernel(SolveArgs a) {
   static_assert(!THINK || ORDER == 2, "think mode needs the evaluator's tables");
   extern __shared__ uint64_t lds[];
   const int bs = blockDim.x, tid = threadIdx.x;

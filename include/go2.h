@@ -68,7 +68,7 @@ typedef struct go2_config {
   int32_t split_min_empties;    /* idle lanes take sibling subtrees of nodes with >= this many
                                    empties (young-brothers-wait); 0 = auto, >60 = never */
   int32_t lds_levels;           /* stack levels (from 4 empties up) kept in LDS, the rest in HBM;
-                                   0 = auto (6) */
+                                   0 = as many as fit a workgroup's LDS share (9 for 256 lanes) */
   int32_t time_limit_ms;        /* abandon a launch after this long (GO2_E_TIMEOUT); 0 = no limit */
   int32_t split_cut_after;      /* at nodes expected to cut, helpers wait for this many children
                                    (1..3) instead of one; 0 = auto */

@@ -23,22 +23,30 @@ def plan_bin(tmp_path_factory):
     return out
 
 
-def plan(binary, n, depth, cus=256, per_cu=3, **fields):
+def plan(binary, n, depth, cus=256, per_cu=2, **fields):
     args = [binary, str(n), str(depth), str(cus), str(per_cu)] + [f"{k}={v}" for k, v in fields.items()]
     r = subprocess.run(args, capture_output=True, text=True, check=True)
     return {k: int(v) for k, v in (ln.split() for ln in r.stdout.splitlines())}
 
 
 def test_headline_shape(plan_bin):
+    # two 256-lane workgroups per CU (the kernel's launch bound), each with
+    # nine stack levels (4..12) in its half of the CU's 160 KB of LDS
     p = plan(plan_bin, 100000, 16)
-    assert (p["depth"], p["maxe"], p["levels"], p["lds_levels"]) == (16, 16, 13, 6)
-    assert p["lds_bytes"] == 6 * 256 * 32 + 2048 + 4 * 64
-    assert (p["grid"], p["waves"], p["total_lanes"]) == (768, 3072, 768 * 256)
-    assert p["K"] == -(-100000 // 3072)
+    assert (p["depth"], p["maxe"], p["levels"], p["lds_levels"]) == (16, 16, 13, 9)
+    assert p["lds_bytes"] == 9 * 256 * 32 + 2048 + 4 * 64
+    assert (p["grid"], p["waves"], p["total_lanes"]) == (512, 2048, 512 * 256)
+    assert p["K"] == -(-100000 // 2048)
     assert (p["help"], p["ext_min"], p["ext_lo"], p["split_min"]) == (1, 12, 9, 8)
-    p = plan(plan_bin, 100000, 16, ordering=2, positions_per_wave=16)
-    assert p["lds_bytes"] == 6 * 256 * 32 + 2048 + 2048 + 4 * 64 and p["K"] == 16
-    assert 3 * p["lds_bytes"] <= 160 * 1024  # three workgroups per CU
+    p = plan(plan_bin, 100000, 16, ordering=2, positions_per_wave=24)
+    assert p["lds_bytes"] == 9 * 256 * 32 + 2048 + 2048 + 4 * 64 and p["K"] == 24
+    assert 2 * p["lds_bytes"] <= 160 * 1024  # two workgroups per CU
+    assert 2 * (p["lds_bytes"] + 256 * 32) > 160 * 1024  # a tenth level would not fit
+    # smaller workgroups: more of them per CU, the same LDS per lane
+    for bs, levels in ((128, 8), (64, 7)):
+        q = plan(plan_bin, 100000, 16, block_size=bs, ordering=2)
+        assert q["lds_levels"] == levels
+        assert (512 // bs) * q["lds_bytes"] <= 160 * 1024
 
 
 def test_depth_beyond_kernel_limit_sizes_for_32(plan_bin):
@@ -93,10 +101,11 @@ NOALLOC = 128 << 20  # GO2_NOALLOC_SCRATCH_BYTES
 
 
 def test_noalloc_scratch_fit(plan_bin):
-    # The no-allocation entry point's scratch holds the full grid up to 18
+    # The no-allocation entry point's scratch holds the full grid up to 24
     # empties; deeper stacks shrink the grid, and every DEPTH fits somehow.
     p = plan(plan_bin, 100000, 16, capacity=NOALLOC)
-    assert p["fits"] == 1 and p["grid"] == 768 and p["help"] == 1
+    assert p["fits"] == 1 and p["grid"] == 512 and p["help"] == 1
+    assert plan(plan_bin, 100000, 24, capacity=NOALLOC)["grid"] == 512
     assert p["scratch_bytes"] <= NOALLOC
     full = plan(plan_bin, 100000, 16)
     assert full["scratch_bytes"] == p["scratch_bytes"]
@@ -105,7 +114,7 @@ def test_noalloc_scratch_fit(plan_bin):
         assert q["fits"] == 1 and q["scratch_bytes"] <= NOALLOC, depth
         assert q["grid"] >= 256, depth  # at least one workgroup per CU
     q = plan(plan_bin, 100000, 32, capacity=NOALLOC)
-    assert q["grid"] < 768  # shrunk
+    assert q["grid"] < 512  # shrunk
     q = plan(plan_bin, 100000, 32, capacity=1 << 20)
     assert q["fits"] == 0
 
