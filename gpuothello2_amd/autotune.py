@@ -5,7 +5,7 @@ Searched knobs (coordinate descent from the library defaults):
   positions_per_wave  boards a wavefront owns: 64 (one lane per board),
                       16 (four lanes per board), 4 (sixteen), 0 (from n)
   block_size          lanes per workgroup
-  lds_levels          search-stack levels kept in LDS (the rest in HBM)
+  lds_levels          search-stack levels kept in LDS (the rest in HBM; 0 = as many as fit)
   split_min_empties   smallest node helper lanes may split
   tt_log2_buckets     transposition table size (deep batches only)
   tt_min_empties      smallest node the table serves (deep batches only)
@@ -26,7 +26,7 @@ from . import Config, Go2Error, Task, default_config
 SPACE = {
     "positions_per_wave": [0, 64, 16, 4],
     "block_size": [256, 128],
-    "lds_levels": [6, 5, 9],
+    "lds_levels": [0, 6, 8],
     "split_min_empties": [8, 7, 10],
 }
 DEEP_SPACE = {"tt_log2_buckets": [0, 26, 29], "tt_min_empties": [12, 10, 8]}

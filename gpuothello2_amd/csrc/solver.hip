@@ -1778,7 +1778,7 @@ int launch_solve(const go2_board* d_boards, size_t n, int max_depth, int8_t* d_s
   args.total_lanes = (unsigned int)total_lanes;
   args.limit_ticks = 0;
   args.max_pops = cfg.max_pops_per_trip > 0 ? cfg.max_pops_per_trip : 2;
-  args.leaf_batch = sp.think || cfg.leaf_batch_min > 64 ? 0 : (cfg.leaf_batch_min > 0 ? cfg.leaf_batch_min : 48);
+  args.leaf_batch = sp.think || cfg.leaf_batch_min > 64 ? 0 : (cfg.leaf_batch_min > 0 ? cfg.leaf_batch_min : 32);
   args.score_batch = sp.think || cfg.score_batch_min > 64 ? 0 : (cfg.score_batch_min > 0 ? cfg.score_batch_min : 64);
   args.asp_width = sp.think || cfg.aspiration_width > 64 ? 0 : 2 * ((cfg.aspiration_width + 1) / 2);
   args.pvs_min = sp.think || cfg.pvs_min_empties <= 0 ? 0 : max(cfg.pvs_min_empties, 5);

@@ -93,7 +93,7 @@ typedef struct go2_config {
                                    evaluator (go2_think) */
   int32_t leaf_batch_min;       /* nodes with three empties queue all their children, and a
                                    wavefront scores queued children together once this many
-                                   (1..64) are waiting; 0 = auto (48), > 64 = each lane scores its
+                                   (1..64) are waiting; 0 = auto (32), > 64 = each lane scores its
                                    own, one child per loop trip */
   int32_t score_batch_min;      /* nodes that order their moves queue every child, and a
                                    wavefront computes the queued ordering keys together once
