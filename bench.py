@@ -6,7 +6,7 @@ positions at 16 empties, seed 20261203, config id 4), boards resident in HBM
 (a go2 task uploaded once), timed from launch to completion.  The timed
 solver is the headline configuration: learned move ordering, a shared
 lock-free transposition table of 2^29 buckets probed at nodes with >= 8
-empties, 32 boards per wavefront.
+empties, 40 boards per wavefront.
 
 Beside the timed steps, in the same run:
   * A/B on the first 8,192 positions of the same batch, three legs timed with
@@ -94,7 +94,7 @@ def main() -> int:
     ap.add_argument("--tt-log2", type=int, default=29,
                     help="headline solver: shared table of 2^N 16-byte buckets (0 = none; 29 = 8 GiB)")
     ap.add_argument("--tt-min", type=int, default=8, help="headline solver: probe/store nodes with >= N empties")
-    ap.add_argument("--ppw", type=int, default=32, help="headline solver: boards a wavefront owns at once")
+    ap.add_argument("--ppw", type=int, default=40, help="headline solver: boards a wavefront owns at once")
     ap.add_argument("--ab-n", type=int, default=8192, help="positions of the A/B legs (0 = skip)")
     ap.add_argument("--ab-limit-s", type=int, default=240, help="time limit of each A/B leg")
     ap.add_argument("--cpu-sample", type=int, default=2048,
