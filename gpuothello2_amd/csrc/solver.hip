@@ -27,19 +27,26 @@
 //     helpers, one level per loop trip) instead of waiting them out.
 //   * The node being searched lives in VGPRs.  Ancestors are spilled to a
 //     stack, one frame per empties level: levels 4..(lds_levels+3) in LDS
-//     (struct-of-arrays, lane index fastest, bank-conflict free), higher
-//     levels in HBM (same layout over all lanes of the grid; touched only
-//     by the rare nodes near the root).  A frame holds the flip set of the
+//     (struct-of-arrays, lane index fastest, bank-conflict free; 4..12 with
+//     the default 256-lane workgroups), higher levels in HBM (same layout
+//     over all lanes of the grid; touched only by the rare nodes near the
+//     root).  A frame holds the flip set of the
 //     move taken (the parent board is rebuilt from the child by XORs), the
 //     untried moves, the packed move order and alpha/beta/best/move/flags.
 //     Frames at split levels add a split word: helpers still out, the best
 //     value they reported, the node's entry alpha and best move (for the
 //     table) and the siblings out with other wavefronts.
-//   * Every loop iteration does the same heavy work on every searching
-//     lane: one flip and one mobility (or, at nodes with three empties, the
-//     exact value of the two-empties child in straight-line code, so nodes
-//     with two empties never become frames).  Move ordering work is spread
-//     over iterations ("scoring" steps: one child per iteration).
+//   * Every loop trip does the same heavy work on every searching lane: one
+//     flip and one mobility.  Work that would leave most lanes idle is
+//     batched across the wavefront instead: a node with three empties
+//     queues all its children and, once enough are queued, all 64 lanes
+//     score queued two-empties children in straight-line code (the leaf
+//     batch); a node that orders its moves queues all its children and the
+//     lanes compute their ordering keys together and rank them by ballots
+//     (the score batch).
+//   * Aspiration windows at board roots (learned ordering) and null-window
+//     searches after the first child from 8 empties, each re-searched when
+//     the bound it returns falls inside the window.
 //   * Across wavefronts: while some wavefront is idle, busy ones export
 //     siblings of high nodes to help slots (more, and lower, as the idle
 //     fraction grows); idle wavefronts claim them and report through
