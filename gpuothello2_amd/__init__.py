@@ -120,12 +120,12 @@ _lib = None
 
 
 def lib() -> ctypes.CDLL:
-    """Load libgo2.so (built in-tree by ``__graft_entry__.build()``)."""
+    """Load libgo2.so (built in-tree by ``make -C gpuothello2_amd``)."""
     global _lib
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
-        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+        raise ImportError(f"{LIB_PATH} is not built; build it with `make -C gpuothello2_amd` first")
     L = ctypes.CDLL(LIB_PATH)
     vp, sz, u64, i32, u32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint32
     L.go2_version.restype = i32
