@@ -1364,6 +1364,7 @@ __global__ void __launch_bounds__(256, 2) solve_kernel(SolveArgs a) {
           const int r = worker ? (int)lmap[lane] : lane;
           const uint64_t jP = shfl64(P, r), jO = shfl64(O, r), jm = shfl64(to_score, r);
           const int jE = __shfl(E, r), joff = __shfl(off, r), jcnt = __shfl(cnt, r);
+          const int jbeta = __shfl(beta, r);
           const int k = lane - joff;
           int key = 0, jsq = 0;
           if (worker) {
@@ -1375,6 +1376,16 @@ __global__ void __launch_bounds__(256, 2) solve_kernel(SolveArgs a) {
             const uint64_t cP = jO ^ f, cO = jP ^ f ^ bit;
             key = move_key_of(cP, cO, mobility(cP, cO), bit, jE);
             ++c_steps;
+            if (a.tt && jE - 1 >= a.tt_min) {
+              // A child whose table entry already proves the node's cutoff
+              // (an upper bound <= -beta) is searched first: its probe then
+              // settles it and the node is cut after one step.
+              const uint64_t h = tt_key(cP, cO);
+              const ulonglong2 e = a.tt[h & a.tt_mask];
+              if ((e.x ^ e.y) == h && (uint32_t)((e.y >> 24) & 255) == tt_gen &&
+                  (int)((e.y >> 8) & 255) - 64 <= -jbeta)
+                key = 0;
+            }
           }
           // Rank of each job inside its node's segment of lanes [joff,
           // joff + jcnt): jobs with a smaller key, then equal keys on lower
