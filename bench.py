@@ -5,7 +5,7 @@ One step = one exact solve of the whole batch (default 100,000 random-playout
 positions at 16 empties, seed 20261203, config id 4), boards resident in HBM
 (a go2 task uploaded once), timed from launch to completion.  The timed
 solver is the headline configuration: learned move ordering, a shared
-lock-free transposition table of 2^29 buckets probed at nodes with >= 8
+lock-free transposition table of 2^31 buckets probed at nodes with >= 8
 empties, 40 boards per wavefront.
 
 Beside the timed steps, in the same run:
@@ -91,8 +91,8 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=100000, help="positions per step")
     ap.add_argument("--empties", type=int, default=16)
     ap.add_argument("--seed", type=int, default=20261203)
-    ap.add_argument("--tt-log2", type=int, default=29,
-                    help="headline solver: shared table of 2^N 16-byte buckets (0 = none; 29 = 8 GiB)")
+    ap.add_argument("--tt-log2", type=int, default=31,
+                    help="headline solver: shared table of 2^N 16-byte buckets (0 = none; 31 = 32 GiB)")
     ap.add_argument("--tt-min", type=int, default=8, help="headline solver: probe/store nodes with >= N empties")
     ap.add_argument("--ppw", type=int, default=40, help="headline solver: boards a wavefront owns at once")
     ap.add_argument("--ab-n", type=int, default=8192, help="positions of the A/B legs (0 = skip)")

@@ -29,7 +29,7 @@ SPACE = {
     "lds_levels": [0, 6, 8],
     "split_min_empties": [8, 7, 10],
 }
-DEEP_SPACE = {"tt_log2_buckets": [0, 26, 29], "tt_min_empties": [12, 10, 8]}
+DEEP_SPACE = {"tt_log2_buckets": [0, 26, 29, 31], "tt_min_empties": [12, 10, 8]}
 
 
 def apply(base: Config | None, params: dict) -> Config:

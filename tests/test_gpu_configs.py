@@ -11,7 +11,7 @@ boards, which each test first checks through the boards' SHA-256.
       static corner / X-square order, no table, no help)
   C3  the first 20,000 of the 10..18-empties mix: autotuned configuration,
       one persistent launch per solve captured in a HIP graph and replayed
-  C4  100,000 x 16 empties: the headline solver (learned order, 2^29-bucket
+  C4  100,000 x 16 empties: the headline solver (learned order, 2^31-bucket
       table from 8 empties, 40 boards per wavefront); learned order without
       a table on the first 8,192
   C5  20 empties with a 2^26-bucket table from 10 empties, learned order
@@ -124,7 +124,7 @@ def test_c4_headline_solver_full_size(evaluator):
     fx = fixture("c4_16e.npz")
     b = boards_of(fx)
     assert len(b) == 100_000
-    cfg = go2.default_config(ordering=go2.ORDER_LEARNED, tt_log2_buckets=29, tt_min_empties=8,
+    cfg = go2.default_config(ordering=go2.ORDER_LEARNED, tt_log2_buckets=31, tt_min_empties=8,
                              positions_per_wave=40, time_limit_ms=TIME_LIMIT_MS)
     [(s, st)] = solve_task(b, 16, cfg)
     log(f"C4 headline: {st['kernel_ms']:.0f} ms, hits {st['tt_hits']} of {st['tt_probes']}")
