@@ -189,6 +189,10 @@ constexpr int kMovePass = 64, kMoveNone = 65;
 constexpr int kLeafAge = 3;
 // ... and a score batch when its oldest node has waited this many.
 constexpr int kScoreAge = 3;
+// Learned ordering scores a child by the evaluator one ply deeper (its
+// replies) at nodes with at least this many empties: about ten evaluations
+// per child instead of one, paid only near the roots of deep boards.
+constexpr int kPlyMin = 15;
 
 struct SolveArgs {
   const go2_board* boards;
@@ -556,15 +560,40 @@ __global__ void __launch_bounds__(256, 2) solve_kernel(SolveArgs a) {
   };
 
   // Ordering key of the child (cP, cO) with replies cm, reached by the move
-  // `bit` from a node with E empties: fastest first (fewest replies),
-  // corners preferred, X-squares delayed; near the root the learned
-  // evaluator of the child joins the key.
+  // `bit` from a node with E empties: fastest first (fewest replies), then
+  // corners, plain squares, X-squares, and inside each class a move into a
+  // 4x4 quadrant that held an odd number of empties (the quadrant now holds
+  // an even number).  From learned_min empties the learned evaluator of the
+  // child joins the key; from kPlyMin empties the child's value is the
+  // evaluator one ply deeper, the opponent's best reply.
   auto move_key_of = [&](uint64_t cP, uint64_t cO, uint64_t cm, uint64_t bit, int e) -> int {
-    int k = min(popc64(cm), 15) * 4 + ((bit & kCorners) ? 0 : 2) + ((bit & kXSquares) ? 1 : 0);
+    const int bsq = ctz64(bit);
+    const uint64_t quad = 0x000000000F0F0F0FULL << ((((bsq >> 2) & 1) << 2) | (((bsq >> 5) & 1) << 5));
+    int k = min(popc64(cm), 8) * 7 + ((bit & kCorners) ? 0 : ((bit & kXSquares) ? 5 : 3)) +
+            (popc64(~(cP | cO) & quad) & 1);
     if constexpr (ORDER == 2) {
       if (e >= a.learned_min) {
-        int ev = a.ev_bias[e - 1];
-        for_each_feature(cP, cO, t3, [&](int, uint32_t fi) { ev += a.ev_w[fi]; });
+        // One evaluation site: the child itself, or each of its replies
+        // (the opponent's value is the largest -eval over them).
+        const bool ply = !THINK && e >= kPlyMin && cm;
+        uint64_t rm = ply ? cm : 1ULL;
+        int lo = 1 << 30;
+        do {
+          uint64_t xP = cP, xO = cO;
+          int bi = e - 1;
+          if (ply) {
+            const int rs = ctz64(rm);
+            const uint64_t rf = flip(cP, cO, rs);
+            xP = cO ^ rf;
+            xO = cP ^ rf ^ (1ULL << rs);
+            bi = e - 2;
+          }
+          rm &= rm - 1;
+          int g = a.ev_bias[bi];
+          for_each_feature(xP, xO, t3, [&](int, uint32_t fi) { g += a.ev_w[fi]; });
+          lo = min(lo, ply ? g : -g);
+        } while (rm);
+        const int ev = -lo;
         k = min(max(((ev * a.key_eval_mul) >> 12) + popc64(cm) * a.key_mob_mul + 32, 0), 63);
       }
     }

@@ -73,7 +73,9 @@ typedef struct go2_config {
   int32_t split_cut_after;      /* at nodes expected to cut, helpers wait for this many children
                                    (1..3) instead of one; 0 = auto */
   int32_t learned_key_eval;     /* GO2_ORDER_LEARNED move key = eval(child)*this/4096 (evaluator
-                                   units) + replies*learned_key_mobility; 0 = auto */
+                                   units) + replies*learned_key_mobility; 0 = auto.  At nodes
+                                   with >= 15 empties eval(child) is one ply deeper: the
+                                   largest -eval over the child's replies */
   int32_t learned_key_mobility;
   int32_t max_pops_per_trip;    /* finished nodes a lane returns from per loop trip; 0 = auto (2) */
   int32_t tt_log2_buckets;      /* transposition table of 2^n 16-byte buckets shared by the whole
