@@ -560,17 +560,24 @@ __global__ void __launch_bounds__(256, 2) solve_kernel(SolveArgs a) {
   };
 
   // Ordering key of the child (cP, cO) with replies cm, reached by the move
-  // `bit` from a node with E empties: fastest first (fewest replies), then
-  // corners, plain squares, X-squares, and inside each class a move into a
-  // 4x4 quadrant that held an odd number of empties (the quadrant now holds
-  // an even number).  From learned_min empties the learned evaluator of the
-  // child joins the key; from kPlyMin empties the child's value is the
-  // evaluator one ply deeper, the opponent's best reply.
+  // `bit` from a node with E empties: fastest first (fewest replies),
+  // corners preferred, X-squares delayed.  Learned ordering below
+  // learned_min empties, where reply counts stay small, caps them at 8 and
+  // orders inside each count by corner / plain / X-square and then a move
+  // into a 4x4 quadrant that held an odd number of empties (the quadrant now
+  // holds an even number).  From learned_min empties the learned evaluator
+  // of the child joins the key; from kPlyMin empties the child's value is
+  // the evaluator one ply deeper, the opponent's best reply.
   auto move_key_of = [&](uint64_t cP, uint64_t cO, uint64_t cm, uint64_t bit, int e) -> int {
-    const int bsq = ctz64(bit);
-    const uint64_t quad = 0x000000000F0F0F0FULL << ((((bsq >> 2) & 1) << 2) | (((bsq >> 5) & 1) << 5));
-    int k = min(popc64(cm), 8) * 7 + ((bit & kCorners) ? 0 : ((bit & kXSquares) ? 5 : 3)) +
-            (popc64(~(cP | cO) & quad) & 1);
+    int k;
+    if (ORDER == 2 && e < a.learned_min) {
+      const int bsq = ctz64(bit);
+      const uint64_t quad = 0x000000000F0F0F0FULL << ((((bsq >> 2) & 1) << 2) | (((bsq >> 5) & 1) << 5));
+      k = min(popc64(cm), 8) * 7 + ((bit & kCorners) ? 0 : ((bit & kXSquares) ? 5 : 3)) +
+          (popc64(~(cP | cO) & quad) & 1);
+    } else {
+      k = min(popc64(cm), 15) * 4 + ((bit & kCorners) ? 0 : 2) + ((bit & kXSquares) ? 1 : 0);
+    }
     if constexpr (ORDER == 2) {
       if (e >= a.learned_min) {
         // One evaluation site: the child itself, or each of its replies
