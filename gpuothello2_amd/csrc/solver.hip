@@ -193,6 +193,10 @@ constexpr int kScoreAge = 3;
 // replies) at nodes with at least this many empties: about ten evaluations
 // per child instead of one, paid only near the roots of deep boards.
 constexpr int kPlyMin = 15;
+// Mobility ordering uses the capped fastest-first key with quadrant-parity
+// ties at nodes with at most this many empties (above it, more than 8
+// replies are common and the uncapped count orders better).
+constexpr int kParityKeyMax = 10;
 
 struct SolveArgs {
   const go2_board* boards;
@@ -561,16 +565,17 @@ __global__ void __launch_bounds__(256, 2) solve_kernel(SolveArgs a) {
 
   // Ordering key of the child (cP, cO) with replies cm, reached by the move
   // `bit` from a node with E empties: fastest first (fewest replies),
-  // corners preferred, X-squares delayed.  Learned ordering below
-  // learned_min empties, where reply counts stay small, caps them at 8 and
-  // orders inside each count by corner / plain / X-square and then a move
-  // into a 4x4 quadrant that held an odd number of empties (the quadrant now
-  // holds an even number).  From learned_min empties the learned evaluator
-  // of the child joins the key; from kPlyMin empties the child's value is
-  // the evaluator one ply deeper, the opponent's best reply.
+  // corners preferred, X-squares delayed.  Where reply counts stay small
+  // (learned ordering below learned_min empties, mobility ordering up to
+  // kParityKeyMax empties) they are capped at 8 and each count is ordered by
+  // corner / plain / X-square and then a move into a 4x4 quadrant that held
+  // an odd number of empties (the quadrant now holds an even number).  From
+  // learned_min empties the learned evaluator of the child joins the key;
+  // from kPlyMin empties the child's value is the evaluator one ply deeper,
+  // the opponent's best reply.
   auto move_key_of = [&](uint64_t cP, uint64_t cO, uint64_t cm, uint64_t bit, int e) -> int {
     int k;
-    if (ORDER == 2 && e < a.learned_min) {
+    if ((ORDER == 2 && e < a.learned_min) || (ORDER == 1 && e <= kParityKeyMax)) {
       const int bsq = ctz64(bit);
       const uint64_t quad = 0x000000000F0F0F0FULL << ((((bsq >> 2) & 1) << 2) | (((bsq >> 5) & 1) << 5));
       k = min(popc64(cm), 8) * 7 + ((bit & kCorners) ? 0 : ((bit & kXSquares) ? 5 : 3)) +
