@@ -571,8 +571,8 @@ __global__ void __launch_bounds__(256, 2) solve_kernel(SolveArgs a) {
   // corner / plain / X-square and then a move into a 4x4 quadrant that held
   // an odd number of empties (the quadrant now holds an even number).  From
   // learned_min empties the learned evaluator of the child joins the key;
-  // from kPlyMin empties the child's value is the evaluator one ply deeper,
-  // the opponent's best reply.
+  // in the exact search (not think mode) from kPlyMin empties the child's
+  // value is the evaluator one ply deeper, the opponent's best reply.
   auto move_key_of = [&](uint64_t cP, uint64_t cO, uint64_t cm, uint64_t bit, int e) -> int {
     int k;
     if ((ORDER == 2 && e < a.learned_min) || (ORDER == 1 && e <= kParityKeyMax)) {
