@@ -197,6 +197,10 @@ constexpr int kPlyMin = 15;
 // ties at nodes with at most this many empties (above it, more than 8
 // replies are common and the uncapped count orders better).
 constexpr int kParityKeyMax = 10;
+// The capped key (replies <= 8, seven classes per count) and the uncapped one
+// (replies <= 15, four per count) stay below 64: the score batch ranks
+// 6-bit keys.
+static_assert(8 * 7 + 5 + 1 <= 63 && 15 * 4 + 3 <= 63, "ordering keys fit 6 bits");
 
 struct SolveArgs {
   const go2_board* boards;
